@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X gravity kick: particle-kicks/s on one GPU.
+
+Workload (default): 256^3 = 16,777,216 particles, 70% in 2000 NFW halos
+(masses log-uniform over 3 decades, concentration 5-15, truncated at r200) and
+30% uniform, seed 42; theta = 0.55, leaf bucket 128, Ewald on, expansion
+order 7, softening (1/80)(693/256) N^(-1/3), GM = 1/N; first kick of the
+top level (kick_params of src/driver.cpp:481-508 with tau = 0).
+
+A step is one kick of every particle of the resident box: the walk, all
+interactions (M2L, Ewald, P2L, P2P, M2P, L2P) and the kick update.  Inputs are
+resident in HBM when the timed region starts; the tree is built once before.
+
+Prints one JSON line.  `roofline` describes the dominant kernel (the leaf
+kernel: P2P + M2P + L2P + kick) with algorithmic flops from the reference's
+own per-event weights; `cpu_baseline` times the CPU oracle (a plain-C FP64
+restatement of the same walk, multithreaded) on a bounded sample of the same
+box on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 figure
+# reference per-event flop weights (src/gravity.cu:343; src/gravity.cu:197-203 with
+# cosmictiger/fmm_kernels.hpp:306,1890; src/kick.cu:145,153)
+W_PP = 21.0
+W_PC = 6 + 262 + 520 + 4
+W_PC_PHI = 119
+W_SINK = 557 + 36
+W_SINK_PHI = 178
+
+
+def make_box(args):
+    from cosmictiger_amd import workloads
+    n = args.n
+    if args.workload == "uniform":
+        x, y, z = workloads.uniform_box(n, args.seed)
+    else:
+        x, y, z = workloads.nfw_box(n, args.seed, clustered_fraction=0.7, nhalos=2000)
+    return x, y, z
+
+
+def leaf_flops(st, do_phi):
+    return (W_PP * st["pp_pairs"] + (W_PC + W_PC_PHI * do_phi) * st["pc_pairs"] + (W_SINK + W_SINK_PHI * do_phi) * st["sinks"])
+
+
+def cpu_baseline(x, y, z, args, h):
+    """Oracle kick on a bounded sample of sinks of the same box (CPU, this host)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ct_oracle as oc
+    nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    tr = oc.OracleTree(x, y, z, args.bucket, args.theta, h, nthreads=nthreads)
+    n = len(x)
+    vel = np.zeros((n, 3), np.float32)
+    rung = np.zeros(n, np.int8)
+    p = oc.default_params(args.theta, h, 1.0 / n)
+    sample = int(args.cpu_sample)
+    start = n // 2 - sample // 2
+    # grow the sample until the timing is at least ~10 s of CPU work or the cap is hit
+    while True:
+        t0 = time.perf_counter()
+        oc.kick(tr, p, vel, rung, want_forces=False, nthreads=nthreads, sample=(start, start + sample))
+        dt = time.perf_counter() - t0
+        if dt * nthreads >= 10.0 or sample >= args.cpu_sample_max or sample >= n:
+            break
+        sample = min(n, sample * 4, args.cpu_sample_max)
+        start = max(0, n // 2 - sample // 2)
+    return {"value": sample / dt, "unit": "particle-kicks/s", "cores": nthreads, "kind": "port",
+            "sample": "oracle kick (plain C FP64 restatement of the walk) of %d sinks [%d, %d) of the same %d-particle box, "
+                      "%.2f s wall on %d threads" % (sample, start, start + sample, n, dt, nthreads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256 ** 3)
+    ap.add_argument("--workload", default="clustered", choices=["clustered", "uniform"])
+    ap.add_argument("--theta", type=float, default=0.55)
+    ap.add_argument("--bucket", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample-max", type=int, default=262144)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "leaf_kernel_traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; runs on %d GPUs are not part of it (boxes shard one per GPU, "
+              "see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import cosmictiger_amd as ct
+    t0 = time.perf_counter()
+    x, y, z = make_box(args)
+    n = len(x)
+    h = ct.softening(n)
+    ct.init(0)
+    box = ct.Box(x, y, z)
+    tb0 = time.perf_counter()
+    box.tree_build(args.theta, args.bucket, h)
+    tree_s = time.perf_counter() - tb0
+    params = ct.default_kick_params(args.theta, h, 1.0 / n)
+    print("bench: %d particles, tree %.2fs, setup %.2fs" % (n, tree_s, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        box.kick(params)
+    leaf_ms = []
+    stats = None
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        box.kick(params)
+        stats = box.stats()
+        leaf_ms.append(stats["leaf_ms"])
+    elapsed = time.perf_counter() - ts
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n * args.steps / elapsed
+    flops = leaf_flops(stats, 1.0 if params.do_phi else 0.0)
+    leaf_s = np.mean(leaf_ms) * 1e-3
+    achieved = flops / leaf_s / 1e12
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        try:
+            with open(args.traffic_file) as f:
+                tr = json.load(f)
+            if tr.get("n") == n and tr.get("workload") == args.workload:
+                traffic = tr.get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(x, y, z, args, h)
+    # drift timing (reported beside the kick)
+    td = time.perf_counter()
+    box.drift(0, 1.0, 1.0 / 64)
+    drift_ms = 1e3 * (time.perf_counter() - td)
+    out = {
+        "metric": "particle-kicks/s at rms force err <=1e-3 vs FP64 Ewald; p50/p99 step latency",
+        "value": value,
+        "unit": "particle-kicks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "%s box %d particles (70%% in 2000 NFW halos, 30%% uniform), theta %.2f, bucket %d, "
+                               "order 7, Ewald, seed %d" % (args.workload, n, args.theta, args.bucket, args.seed)
+                   if args.workload == "clustered" else "uniform box %d particles, theta %.2f, bucket %d, order 7, Ewald, seed %d"
+                   % (n, args.theta, args.bucket, args.seed),
+                   "n": n, "theta": args.theta, "bucket": args.bucket, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": "k_leaf (P2P + M2P + L2P + kick), FP32 vector ALU; peak = MI355X FP32 figure",
+                     "flops_per_launch": flops, "launch_ms": float(np.mean(leaf_ms))},
+        "cpu_baseline": cpu,
+        "phases_ms": {k: stats[k] for k in ("walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "kick_ms")},
+        "events": {k: stats[k] for k in ("pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs", "nodes", "sinks")},
+        "tree_build_s": tree_s,
+        "drift_ms": drift_ms,
+    }
+    box.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,305 @@
+"""MI355X-native CosmicTiger gravity kick: Python host mirror of the C-ABI.
+
+This module binds ``libcosmictiger_amd.so`` (include/cosmictiger_amd.h) with
+ctypes and mirrors the reference's host interface for the kick path:
+
+* ``KickParams`` / ``KickReturn``: ``kick_params`` / ``kick_return``
+  (cosmictiger/kick.hpp:92-186), same fields, same order.
+* ``Box``: a resident box of particles (cosmictiger/particles.hpp:159-306)
+  with ``tree_build`` (tree_create, src/tree.cpp:273), ``kick``
+  (kick(params, 0, 0, root, {root}, {root}), src/kick.cpp:150) and ``drift``
+  (cuda_drift, src/drift.cu:156).
+
+The library must be built in-tree (``cosmictiger_amd.build``); importing
+``lib()`` without it raises, there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcosmictiger_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cosmictiger_amd.h")
+
+CT_OK = 0
+CT_E_ARG = -1
+CT_E_OOM = -2
+CT_E_HIP = -3
+CT_E_ARENA = -4
+CT_E_STATE = -5
+CT_E_TREE = -6
+CT_E_WALK = -7
+
+SELF_PHI = 693.0 / 256.0  # cosmictiger/gravity.hpp:98-102
+EXPANSION_SIZE = 50
+MULTIPOLE_SIZE = 37
+
+_lib = None
+_inited_device = None
+
+
+class CTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+class KickParams(ctypes.Structure):
+    """kick_params (cosmictiger/kick.hpp:138-157)."""
+    _fields_ = [("sign", ctypes.c_float), ("min_level", ctypes.c_int), ("min_rung", ctypes.c_int), ("a", ctypes.c_float),
+                ("t0", ctypes.c_float), ("theta", ctypes.c_float), ("h", ctypes.c_float), ("eta", ctypes.c_float),
+                ("GM", ctypes.c_float), ("vel_init", ctypes.c_bool), ("save_force", ctypes.c_bool),
+                ("first_call", ctypes.c_bool), ("gpu", ctypes.c_bool), ("node_load", ctypes.c_float),
+                ("max_dt", ctypes.c_float), ("ascending", ctypes.c_bool), ("descending", ctypes.c_bool),
+                ("top", ctypes.c_bool), ("do_phi", ctypes.c_bool)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        # defaults of the reference constructor (kick.hpp:158-163)
+        self.max_dt = 1e30
+        self.do_phi = True
+        self.vel_init = False
+        self.sign = 1.0
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class KickReturn(ctypes.Structure):
+    """kick_return (cosmictiger/kick.hpp:92-100)."""
+    _fields_ = [("max_rung", ctypes.c_int), ("pot", ctypes.c_double), ("kin", ctypes.c_double), ("dkin", ctypes.c_double),
+                ("xmom", ctypes.c_double), ("ymom", ctypes.c_double), ("zmom", ctypes.c_double), ("nmom", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class Node(ctypes.Structure):
+    _fields_ = [("box_begin", ctypes.c_int32 * 3), ("box_end", ctypes.c_int32 * 3), ("child", ctypes.c_int32 * 2),
+                ("parent", ctypes.c_int32), ("pbegin", ctypes.c_int32), ("pend", ctypes.c_int32), ("pos", ctypes.c_uint32 * 3),
+                ("radius", ctypes.c_float), ("leaf", ctypes.c_int32), ("depth", ctypes.c_int32), ("multi", ctypes.c_float * 37)]
+
+
+NODE_DTYPE = np.dtype([("box_begin", "<i4", 3), ("box_end", "<i4", 3), ("child", "<i4", 2), ("parent", "<i4"), ("pbegin", "<i4"),
+                       ("pend", "<i4"), ("pos", "<u4", 3), ("radius", "<f4"), ("leaf", "<i4"), ("depth", "<i4"),
+                       ("multi", "<f4", 37)])
+assert NODE_DTYPE.itemsize == ctypes.sizeof(Node)
+
+_P = ctypes.c_void_p
+_I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+def _declare(L):
+    L.ct_init.argtypes = [ctypes.c_int]
+    L.ct_ewald_init.argtypes = []
+    L.ct_box_create.argtypes = [ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]
+    L.ct_box_destroy.argtypes = [_P]
+    L.ct_tree_build.argtypes = [_P, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.ct_tree_build_host.argtypes = [ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_double, _P,
+                                     ctypes.c_int64, _I64P]
+    L.ct_kick.argtypes = [_P, ctypes.POINTER(KickParams), ctypes.POINTER(KickReturn), _P]
+    L.ct_drift.argtypes = [_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_double, _P]
+    L.ct_box_size.argtypes = [_P, _I64P, _I64P]
+    L.ct_box_read.argtypes = [_P] + [_P] * 10
+    L.ct_tree_read.argtypes = [_P, _P, ctypes.c_int64]
+    L.ct_lists_read.argtypes = [_P, ctypes.c_int, _P, _P, ctypes.c_int64]
+    L.ct_kick_stats.argtypes = [_P, _P, ctypes.c_int]
+    L.ct_fmm_host.argtypes = [ctypes.c_int, _P, _P]
+    L.ct_fmm_device.argtypes = [ctypes.c_int, ctypes.c_int64, _P, _P]
+    L.ct_last_error.argtypes = []
+    L.ct_last_error.restype = ctypes.c_char_p
+    for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_host", "ct_kick",
+                 "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
+                 "ct_fmm_device"):
+        getattr(L, name).restype = ctypes.c_int
+
+
+def lib():
+    """The loaded C-ABI library.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcosmictiger_amd.so is not built; run cosmictiger_amd.build.build() "
+                               "(or __graft_entry__.build())")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _check(rc):
+    if rc != CT_OK:
+        raise CTError(rc, lib().ct_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def init(device=0):
+    """Select the device and upload the Ewald tables (ewald_const::init)."""
+    global _inited_device
+    if _inited_device != device:
+        _check(lib().ct_init(device))
+        _inited_device = device
+
+
+# ---------------------------------------------------------------------------
+# expansion kernels (for layout checks)
+# ---------------------------------------------------------------------------
+
+FMM_OPS = {"greens": 0, "ewald_greens": 1, "M2L": 2, "L2L": 3, "L2P": 4, "P2M": 5, "M2M": 6, "M2L2": 7}
+_IN_STRIDE = {0: 3, 1: 3, 2: 87, 3: 53, 4: 53, 5: 3, 6: 40, 7: 87}
+_OUT_STRIDE = {0: 50, 1: 50, 2: 50, 3: 50, 4: 4, 5: 37, 6: 37, 7: 4}
+
+
+def fmm_host(op, inp):
+    """Evaluate one expansion kernel on the host in double."""
+    op = FMM_OPS.get(op, op)
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    assert inp.size == _IN_STRIDE[op]
+    out = np.zeros(_OUT_STRIDE[op])
+    _check(lib().ct_fmm_host(op, _ptr(inp), _ptr(out)))
+    return out
+
+
+def fmm_device(op, inp):
+    """Evaluate one expansion kernel on the device in float for a batch (n, in_stride)."""
+    op = FMM_OPS.get(op, op)
+    inp = np.ascontiguousarray(inp, dtype=np.float64).reshape(-1, _IN_STRIDE[op])
+    out = np.zeros((inp.shape[0], _OUT_STRIDE[op]))
+    _check(lib().ct_fmm_device(op, inp.shape[0], _ptr(inp), _ptr(out)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# tree (host)
+# ---------------------------------------------------------------------------
+
+def tree_build_host(x, y, z, theta, bucket, hsoft):
+    """Host tree build over copies of the positions.  Returns (x, y, z, perm, nodes)."""
+    x = np.array(x, dtype=np.uint32)
+    y = np.array(y, dtype=np.uint32)
+    z = np.array(z, dtype=np.uint32)
+    n = len(x)
+    perm = np.zeros(n, dtype=np.int32)
+    nn = ctypes.c_int64(0)
+    cap = max(1024, 8 * n // max(bucket, 1) + 65536)
+    for _ in range(4):
+        nodes = np.zeros(cap, dtype=NODE_DTYPE)
+        xs, ys, zs = x.copy(), y.copy(), z.copy()
+        rc = lib().ct_tree_build_host(n, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(perm), float(theta), int(bucket), float(hsoft),
+                                      _ptr(nodes), cap, ctypes.byref(nn))
+        if rc == CT_E_ARG and nn.value > cap:
+            cap = nn.value
+            continue
+        _check(rc)
+        return xs, ys, zs, perm, nodes[:nn.value].copy()
+    raise CTError(CT_E_ARG, "tree_build_host: capacity")
+
+
+# ---------------------------------------------------------------------------
+# resident box
+# ---------------------------------------------------------------------------
+
+class Box:
+    """A resident box: particles and tree in HBM, kicked and drifted there."""
+
+    def __init__(self, x, y, z, vel=None, rung=None, device=0):
+        init(device)
+        self.n = len(x)
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        y = np.ascontiguousarray(y, dtype=np.uint32)
+        z = np.ascontiguousarray(z, dtype=np.uint32)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(self.n, 3)
+        rung = None if rung is None else np.ascontiguousarray(rung, dtype=np.int8)
+        h = _P()
+        _check(lib().ct_box_create(self.n, _ptr(x), _ptr(y), _ptr(z), _ptr(vel), _ptr(rung), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ct_box_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tree_build(self, theta, bucket, hsoft):
+        _check(lib().ct_tree_build(self._h, float(theta), int(bucket), float(hsoft)))
+
+    def kick(self, params, stream=None):
+        ret = KickReturn()
+        _check(lib().ct_kick(self._h, ctypes.byref(params), ctypes.byref(ret), stream))
+        return ret
+
+    def drift(self, rung, a, dt, drag=0.0, stream=None):
+        _check(lib().ct_drift(self._h, int(rung), float(a), float(dt), float(drag), stream))
+
+    def size(self):
+        n = ctypes.c_int64()
+        nn = ctypes.c_int64()
+        _check(lib().ct_box_size(self._h, ctypes.byref(n), ctypes.byref(nn)))
+        return n.value, nn.value
+
+    def read(self):
+        n = self.n
+        out = {"x": np.zeros(n, np.uint32), "y": np.zeros(n, np.uint32), "z": np.zeros(n, np.uint32),
+               "vel": np.zeros((n, 3), np.float32), "rung": np.zeros(n, np.int8), "gx": np.zeros(n, np.float32),
+               "gy": np.zeros(n, np.float32), "gz": np.zeros(n, np.float32), "phi": np.zeros(n, np.float32),
+               "id": np.zeros(n, np.int32)}
+        _check(lib().ct_box_read(self._h, *[_ptr(out[k]) for k in ("x", "y", "z", "vel", "rung", "gx", "gy", "gz", "phi", "id")]))
+        return out
+
+    def nodes(self):
+        _, nn = self.size()
+        arr = np.zeros(nn, dtype=NODE_DTYPE)
+        _check(lib().ct_tree_read(self._h, _ptr(arr), nn))
+        return arr
+
+    def lists(self, kind):
+        _, nn = self.size()
+        off = np.zeros(nn + 1, np.int64)
+        _check(lib().ct_lists_read(self._h, int(kind), _ptr(off), None, 0))
+        idx = np.zeros(max(int(off[-1]), 1), np.int32)
+        _check(lib().ct_lists_read(self._h, int(kind), _ptr(off), _ptr(idx), len(idx)))
+        return off, idx[:int(off[-1])]
+
+    def stats(self):
+        out = np.zeros(12)
+        _check(lib().ct_kick_stats(self._h, _ptr(out), 12))
+        keys = ["walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "kick_ms", "pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs",
+                "nodes", "sinks"]
+        return dict(zip(keys, out.tolist()))
+
+
+def default_kick_params(theta, h, GM, eta=None, do_phi=True, save_force=True):
+    """kick_params of the top-level first kick (src/driver.cpp:481-508 with tau = 0)."""
+    p = KickParams()
+    p.sign = 1.0
+    p.min_level = 0
+    p.min_rung = 0
+    p.a = 1.0
+    p.t0 = 1.0
+    p.theta = theta
+    p.h = h
+    p.eta = eta if eta is not None else 0.25 / np.sqrt(SELF_PHI)
+    p.GM = GM
+    p.save_force = save_force
+    p.first_call = True
+    p.gpu = True
+    p.node_load = 0.0
+    p.max_dt = 1e30
+    p.ascending = True
+    p.descending = True
+    p.top = True
+    p.do_phi = do_phi
+    return p
+
+
+def softening(n):
+    """Derived softening of the reference options: (1/80) * self_phi * N^(-1/3)
+    (src/options.cpp:124,165,217)."""
+    return (1.0 / 80.0) * SELF_PHI * n ** (-1.0 / 3.0)

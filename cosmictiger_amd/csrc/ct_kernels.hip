@@ -1,0 +1,1712 @@
+// MI355X (gfx950) kernels of the gravity kick and the drift, and the C-ABI.
+//
+// Kick = five stages, all on one stream, no host synchronisation inside the
+// walk:
+//   1. k_walk (one launch per tree level): the dual-list walk of
+//      src/kick.cu:239-484, done level-synchronously.  One wave64 per sink
+//      node classifies its direct and Ewald check lists (two passes: count,
+//      then write in order), reserves output space with one atomic per list,
+//      and emits the interaction lists plus the check lists of the next level.
+//      Leaves that still hold unresolved checks stay in the work set, which is
+//      the reference's do { } while (checks && leaf) loop (src/kick.cu:344-416).
+//   2. k_cells_ewald / k_cells_direct: one thread per node, M2L over its Ewald
+//      and direct CC lists and P2L over its CP list (src/gravity.cu:44-155,
+//      213-244).
+//   3. k_l2l: per level, L(node) += L2L(L(parent)) (src/kick.cu:246-258).
+//   4. k_leaf: one wave64 per leaf: P2P with sources staged in LDS, M2P, L2P
+//      and the kick update (src/gravity.cu:157-211,246-404; src/kick.cu:53-177).
+//   5. k_reduce: deterministic FP64 reduction of the per-leaf kick_return.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmictiger_amd.h"
+#include "ct_device.hpp"
+#include "ct_expansions.hpp"
+#include "ct_tree.hpp"
+
+namespace ct {
+
+// ---------------------------------------------------------------------------
+// constants
+// ---------------------------------------------------------------------------
+
+constexpr int kMinPartsPCCP = 38;  // src/kick.cu:32
+constexpr float kSelfPhi = 693.0f / 256.0f;  // cosmictiger/gravity.hpp:98-102
+constexpr int kNRealMax = 256;
+constexpr int kNFourMax = 64;
+
+__constant__ float c_rimg[3 * kNRealMax];
+__constant__ float c_fh[3 * kNFourMax];
+__constant__ float c_fc[kNFourMax * 50];
+__constant__ int c_nreal;
+__constant__ int c_nfour;
+__constant__ float c_rung_dt[16];
+
+enum { K_NONE = 0, K_CC = 1, K_CP = 2, K_PC = 3, K_LEAF = 4, K_NEXT = 5, K_FAIL = 6 };
+
+// ---------------------------------------------------------------------------
+// fixed32 and acceptance tests (cosmictiger/fixed.hpp:303-351,
+// cosmictiger/range.hpp:444-466, src/kick.cu:286-386)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float fdist(uint32_t a, uint32_t b) {
+	return float(int32_t(a - b)) * 2.3283064365386963e-10f;
+}
+
+__device__ __forceinline__ float rf_distance(int32_t a, uint32_t b) {
+	float f = float(double(a) * (1.0 / 536870912.0) - double(b) * (1.0 / 4294967296.0));
+	f = float(double(f) - double(f > 0.5f));
+	f = float(double(f) + double(f < -0.5f));
+	return f;
+}
+
+__device__ __forceinline__ bool box_intersects_sphere(const int32_t* bb, const int32_t* be, const uint32_t* x, float r) {
+	float d2 = 0.0f;
+	for (int d = 0; d < 3; d++) {
+		const float db = -rf_distance(bb[d], x[d]);
+		const float de = -rf_distance(be[d], x[d]);
+		const int32_t rfx = int32_t(x[d] >> 3);
+		const int outside = (rfx < bb[d]) + (rfx > be[d]);
+		if (outside) {
+			d2 += float(outside) * fminf(db * db, de * de);
+		}
+	}
+	return d2 < r * r;
+}
+
+__device__ __forceinline__ float mac_r2(const DNode& a, const DNode& b) {
+	const float dx = fdist(a.pos[0], b.pos[0]);
+	const float dy = fdist(a.pos[1], b.pos[1]);
+	const float dz = fdist(a.pos[2], b.pos[2]);
+	return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
+__device__ __forceinline__ int classify_ewald(const DNode& s, const DNode& o, float thinv) {
+	if (o.pend - o.pbegin == 0) return K_NONE;
+	float R2 = mac_r2(s, o);
+	const float m = fmaxf(0.5f - (s.radius + o.radius), 0.f);
+	R2 = fmaxf(R2, m * m);
+	const float dcc = (s.radius + o.radius) * thinv;
+	if (R2 > dcc * dcc) return K_CC;
+	return (o.flags & 1) ? K_FAIL : K_NEXT;
+}
+
+__device__ __forceinline__ int classify_direct(const DNode& s, const DNode& o, float thinv, float h) {
+	if (o.pend - o.pbegin == 0) return K_NONE;
+	const float R2 = mac_r2(s, o);
+	const float rs = s.radius, ro = o.radius;
+	const float mind = (rs + ro) + h;
+	const float dcc = fmaxf((rs + ro) * thinv, mind);
+	if (R2 > dcc * dcc) return K_CC;
+	const bool oleaf = o.flags & 1;
+	if (oleaf && (s.flags & 1)) {
+		const float dcp = fmaxf(__builtin_fmaf(rs, thinv, ro), mind);
+		const float dpc = fmaxf(__builtin_fmaf(ro, thinv, rs), mind);
+		const int sp = s.pend - s.pbegin, op = o.pend - o.pbegin;
+		bool pc = (R2 > dpc * dpc || !box_intersects_sphere(s.bb, s.be, o.pos, fmaxf(thinv * ro, h))) && sp >= kMinPartsPCCP;
+		bool cp = (R2 > dcp * dcp || !box_intersects_sphere(o.bb, o.be, s.pos, fmaxf(thinv * rs, h))) && op >= kMinPartsPCCP;
+		if (pc && cp) {
+			if (sp < op) {
+				cp = false;
+			} else if (sp > op) {
+				pc = false;
+			} else {
+				pc = cp = false;
+			}
+		}
+		if (pc) return K_PC;
+		if (cp) return K_CP;
+	}
+	return oleaf ? K_LEAF : K_NEXT;
+}
+
+__device__ __forceinline__ int lane_id() {
+	return __lane_id();
+}
+
+__device__ __forceinline__ int prefix_below(unsigned long long mask) {
+	return __builtin_amdgcn_mbcnt_hi((unsigned) (mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) mask, 0));
+}
+
+__device__ __forceinline__ int uniform(int x) {
+	return __builtin_amdgcn_readfirstlane(x);
+}
+
+// ---------------------------------------------------------------------------
+// walk
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool reserve_ok(Counters* C, int bit, int64_t off, int64_t cnt, int64_t cap) {
+	if (off + cnt > cap) {
+		atomicOr(&C->overflow, 1 << bit);
+		return false;
+	}
+	return true;
+}
+
+__global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
+	Counters* C = A.cnt;
+	if (__builtin_amdgcn_readfirstlane(C->overflow | C->walk_error)) return;
+	const int nact = uniform(min(C->act[k], int(A.cap_act)));
+	const int lane = lane_id();
+	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+	const int nwaves = (gridDim.x * blockDim.x) >> 6;
+	const Act* act = A.act[k & 1];
+	const int32_t* dp = A.dp[k & 1];
+	const int32_t* ep = A.ep[k & 1];
+	Act* nact_out = A.act[(k + 1) & 1];
+	int32_t* ndp = A.dp[(k + 1) & 1];
+	int32_t* nep = A.ep[(k + 1) & 1];
+	const float thinv = A.theta_inv;
+	const float h = A.h;
+	for (int ai = wave; ai < nact; ai += nwaves) {
+		const int self = uniform(act[ai].self);
+		const int db = uniform(act[ai].db), dn = uniform(act[ai].dn);
+		const int eb = uniform(act[ai].eb), en = uniform(act[ai].en);
+		const DNode s = A.nodes[self];
+		const bool sleaf = s.flags & 1;
+		if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
+		// ---- pass 1: counts
+		int n_ecc = 0, n_enext = 0, n_efail = 0;
+		for (int base = 0; base < en; base += 64) {
+			const int i = base + lane;
+			int cat = K_NONE;
+			if (i < en) {
+				const DNode o = A.nodes[ep[eb + i]];
+				cat = classify_ewald(s, o, thinv);
+			}
+			n_ecc += __popcll(__ballot(cat == K_CC));
+			n_enext += __popcll(__ballot(cat == K_NEXT));
+			n_efail += __popcll(__ballot(cat == K_FAIL));
+		}
+		int n_cc = 0, n_cp = 0, n_pc = 0, n_lf = 0, n_nx = 0;
+		for (int base = 0; base < dn; base += 64) {
+			const int i = base + lane;
+			int cat = K_NONE;
+			if (i < dn) {
+				const DNode o = A.nodes[dp[db + i]];
+				cat = classify_direct(s, o, thinv, h);
+			}
+			n_cc += __popcll(__ballot(cat == K_CC));
+			n_cp += __popcll(__ballot(cat == K_CP));
+			n_pc += __popcll(__ballot(cat == K_PC));
+			n_lf += __popcll(__ballot(cat == K_LEAF));
+			n_nx += __popcll(__ballot(cat == K_NEXT));
+		}
+		if (n_efail || (sleaf && n_enext)) {
+			if (lane == 0) atomicOr(&C->walk_error, 1);
+			continue;
+		}
+		// ---- reservations
+		const int cdn = sleaf ? 2 * n_nx : 2 * n_nx + n_lf;  // direct checks per next entry
+		const int cen = sleaf ? 0 : 2 * n_enext;             // Ewald checks per next entry
+		const int nnext = sleaf ? (n_nx ? 1 : 0) : 2;
+		const int n_pp = sleaf ? n_lf : 0;
+		int r_e = 0, r_cc = 0, r_cp = 0, r_pc = 0, r_pp = 0, r_act = 0, r_dp = 0, r_ep = 0, r_seg = 0;
+		const int nseg = (n_ecc > 0) + (n_cc > 0) + (n_cp > 0) + (n_pc > 0) + (n_pp > 0);
+		if (lane == 0) {
+			if (n_ecc) r_e = atomicAdd(&C->out[kEwald], n_ecc);
+			if (n_cc) r_cc = atomicAdd(&C->out[kCC], n_cc);
+			if (n_cp) r_cp = atomicAdd(&C->out[kCP], n_cp);
+			if (n_pc) r_pc = atomicAdd(&C->out[kPC], n_pc);
+			if (n_pp) r_pp = atomicAdd(&C->out[kPP], n_pp);
+			if (nnext) {
+				r_act = atomicAdd(&C->act[k + 1], nnext);
+				if (cdn * nnext) r_dp = atomicAdd(&C->dp[k + 1], cdn * nnext);
+				if (cen * nnext) r_ep = atomicAdd(&C->ep[k + 1], cen * nnext);
+			}
+			if (nseg) r_seg = atomicAdd(&C->seg, nseg);
+		}
+		r_e = uniform(r_e);
+		r_cc = uniform(r_cc);
+		r_cp = uniform(r_cp);
+		r_pc = uniform(r_pc);
+		r_pp = uniform(r_pp);
+		r_act = uniform(r_act);
+		r_dp = uniform(r_dp);
+		r_ep = uniform(r_ep);
+		r_seg = uniform(r_seg);
+		bool ok_e = true, ok_cc = true, ok_cp = true, ok_pc = true, ok_pp = true, ok_dp = true, ok_ep = true, ok_act = true, ok_seg = true;
+		if (lane == 0) {
+			ok_e = reserve_ok(C, 0, r_e, n_ecc, A.cap_out[kEwald]);
+			ok_cc = reserve_ok(C, 1, r_cc, n_cc, A.cap_out[kCC]);
+			ok_cp = reserve_ok(C, 2, r_cp, n_cp, A.cap_out[kCP]);
+			ok_pc = reserve_ok(C, 3, r_pc, n_pc, A.cap_out[kPC]);
+			ok_pp = reserve_ok(C, 4, r_pp, n_pp, A.cap_out[kPP]);
+			ok_act = reserve_ok(C, 5, r_act, nnext, A.cap_act);
+			ok_dp = reserve_ok(C, 6, r_dp, int64_t(cdn) * nnext, A.cap_dp);
+			ok_ep = reserve_ok(C, 7, r_ep, int64_t(cen) * nnext, A.cap_ep);
+			ok_seg = reserve_ok(C, 8, r_seg, nseg, A.cap_seg);
+		}
+		ok_e = uniform(ok_e);
+		ok_cc = uniform(ok_cc);
+		ok_cp = uniform(ok_cp);
+		ok_pc = uniform(ok_pc);
+		ok_pp = uniform(ok_pp);
+		ok_act = uniform(ok_act);
+		ok_dp = uniform(ok_dp);
+		ok_ep = uniform(ok_ep);
+		ok_seg = uniform(ok_seg);
+		// ---- pass 2: Ewald writes
+		{
+			int run_cc = 0, run_nx = 0;
+			for (int base = 0; base < en; base += 64) {
+				const int i = base + lane;
+				int cat = K_NONE;
+				int oi = 0;
+				DNode o;
+				if (i < en) {
+					oi = ep[eb + i];
+					o = A.nodes[oi];
+					cat = classify_ewald(s, o, thinv);
+				}
+				const unsigned long long mcc = __ballot(cat == K_CC);
+				const unsigned long long mnx = __ballot(cat == K_NEXT);
+				if (cat == K_CC && ok_e) {
+					A.out[kEwald][r_e + run_cc + prefix_below(mcc)] = oi;
+				}
+				if (cat == K_NEXT && ok_ep) {
+					const int pos = 2 * (run_nx + prefix_below(mnx));
+					nep[r_ep + pos] = o.child[0];
+					nep[r_ep + pos + 1] = o.child[1];
+					nep[r_ep + cen + pos] = o.child[0];
+					nep[r_ep + cen + pos + 1] = o.child[1];
+				}
+				run_cc += __popcll(mcc);
+				run_nx += __popcll(mnx);
+			}
+		}
+		// ---- pass 2: direct writes
+		{
+			int run_cc = 0, run_cp = 0, run_pc = 0, run_lf = 0, run_nx = 0;
+			for (int base = 0; base < dn; base += 64) {
+				const int i = base + lane;
+				int cat = K_NONE;
+				int oi = 0;
+				DNode o;
+				if (i < dn) {
+					oi = dp[db + i];
+					o = A.nodes[oi];
+					cat = classify_direct(s, o, thinv, h);
+				}
+				const unsigned long long mcc = __ballot(cat == K_CC);
+				const unsigned long long mcp = __ballot(cat == K_CP);
+				const unsigned long long mpc = __ballot(cat == K_PC);
+				const unsigned long long mlf = __ballot(cat == K_LEAF);
+				const unsigned long long mnx = __ballot(cat == K_NEXT);
+				if (cat == K_CC && ok_cc) {
+					A.out[kCC][r_cc + run_cc + prefix_below(mcc)] = oi;
+				} else if (cat == K_CP && ok_cp) {
+					A.out[kCP][r_cp + run_cp + prefix_below(mcp)] = oi;
+				} else if (cat == K_PC && ok_pc) {
+					A.out[kPC][r_pc + run_pc + prefix_below(mpc)] = oi;
+				} else if (cat == K_LEAF) {
+					const int rank = run_lf + prefix_below(mlf);
+					if (sleaf) {
+						if (ok_pp) A.out[kPP][r_pp + rank] = oi;
+					} else if (ok_dp) {
+						const int pos = 2 * n_nx + rank;
+						ndp[r_dp + pos] = oi;
+						ndp[r_dp + cdn + pos] = oi;
+					}
+				} else if (cat == K_NEXT && ok_dp) {
+					const int pos = 2 * (run_nx + prefix_below(mnx));
+					ndp[r_dp + pos] = o.child[0];
+					ndp[r_dp + pos + 1] = o.child[1];
+					if (!sleaf) {
+						ndp[r_dp + cdn + pos] = o.child[0];
+						ndp[r_dp + cdn + pos + 1] = o.child[1];
+					}
+				}
+				run_cc += __popcll(mcc);
+				run_cp += __popcll(mcp);
+				run_pc += __popcll(mpc);
+				run_lf += __popcll(mlf);
+				run_nx += __popcll(mnx);
+			}
+		}
+		// ---- next work entries and list segments
+		if (lane == 0) {
+			if (nnext && ok_act && ok_dp && ok_ep) {
+				if (sleaf) {
+					Act e;
+					e.self = self;
+					e.db = r_dp;
+					e.dn = cdn;
+					e.eb = 0;
+					e.en = 0;
+					nact_out[r_act] = e;
+				} else {
+					Act e;
+					e.self = s.child[0];
+					e.db = r_dp;
+					e.dn = cdn;
+					e.eb = r_ep;
+					e.en = cen;
+					nact_out[r_act] = e;
+					e.self = s.child[1];
+					e.db = r_dp + cdn;
+					e.eb = r_ep + cen;
+					nact_out[r_act + 1] = e;
+				}
+			}
+			if (ok_seg) {
+				int sid = r_seg;
+				const int kinds[5] = {kEwald, kCC, kCP, kPC, kPP};
+				const int offs[5] = {r_e, r_cc, r_cp, r_pc, r_pp};
+				const int cnts[5] = {n_ecc, n_cc, n_cp, n_pc, n_pp};
+				for (int q = 0; q < 5; q++) {
+					if (cnts[q] == 0) continue;
+					Seg sg;
+					sg.begin = offs[q];
+					sg.len = cnts[q];
+					int32_t* hp = A.head + int64_t(kinds[q]) * A.nnodes + self;
+					sg.next = *hp;
+					sg.pad = 0;
+					A.seg[sid] = sg;
+					*hp = sid;
+					sid++;
+				}
+			}
+		}
+	}
+}
+
+struct InitArgs {
+	int32_t* head;
+	int64_t nhead;
+	Counters* cnt;
+	Act* act0;
+	int32_t* dp0;
+	int32_t* ep0;
+	int32_t root;
+};
+
+__global__ void k_walk_init(InitArgs A) {
+	const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+	for (int64_t j = i; j < A.nhead; j += stride) {
+		A.head[j] = -1;
+	}
+	int32_t* c = reinterpret_cast<int32_t*>(A.cnt);
+	const int64_t nc = sizeof(Counters) / sizeof(int32_t);
+	for (int64_t j = i; j < nc; j += stride) {
+		c[j] = 0;
+	}
+	if (i == 0) {
+		Act e;
+		e.self = A.root;
+		e.db = 0;
+		e.dn = 1;
+		e.eb = 0;
+		e.en = 1;
+		e.pad[0] = e.pad[1] = e.pad[2] = 0;
+		A.act0[0] = e;
+		A.dp0[0] = A.root;
+		A.ep0[0] = A.root;
+	}
+}
+
+__global__ void k_walk_seed_counts(Counters* C) {
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		C->act[0] = 1;
+		C->dp[0] = 1;
+		C->ep[0] = 1;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// cell interactions: thread per node
+// ---------------------------------------------------------------------------
+
+struct CellArgs {
+	const DNode* nodes;
+	const float* multi;
+	float* L;
+	const int32_t* head;
+	const Seg* seg;
+	const int32_t* out_e;
+	const int32_t* out_cc;
+	const int32_t* out_cp;
+	const uint32_t* px;
+	const uint32_t* py;
+	const uint32_t* pz;
+	int32_t nnodes;
+	bool do_phi;
+	Counters* cnt;
+};
+
+__device__ __forceinline__ void load_multi(const float* __restrict__ mp, float* M, uint32_t* pos) {
+	const float4* m4 = reinterpret_cast<const float4*>(mp);
+	float tmp[40];
+#pragma unroll
+	for (int q = 0; q < 10; q++) {
+		const float4 v = m4[q];
+		tmp[4 * q + 0] = v.x;
+		tmp[4 * q + 1] = v.y;
+		tmp[4 * q + 2] = v.z;
+		tmp[4 * q + 3] = v.w;
+	}
+#pragma unroll
+	for (int q = 0; q < 37; q++) {
+		M[q] = tmp[q];
+	}
+	pos[0] = __float_as_uint(tmp[37]);
+	pos[1] = __float_as_uint(tmp[38]);
+	pos[2] = __float_as_uint(tmp[39]);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+	for (int o = 32; o > 0; o >>= 1) {
+		v += __shfl_xor(v, o);
+	}
+	return v;
+}
+
+__global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	unsigned long long nev = 0;
+	if (i < A.nnodes) {
+		const DNode s = A.nodes[i];
+		float L[50];
+#pragma unroll
+		for (int q = 0; q < 50; q++) L[q] = 0.0f;
+		for (int sg = A.head[int64_t(kEwald) * A.nnodes + i]; sg >= 0; sg = A.seg[sg].next) {
+			const Seg S = A.seg[sg];
+			for (int j = 0; j < S.len; j++) {
+				const int o = A.out_e[S.begin + j];
+				float M[37];
+				uint32_t op[3];
+				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
+				float D[50];
+				ctfmm::ewald_greens<float>(D, fdist(s.pos[0], op[0]), fdist(s.pos[1], op[1]), fdist(s.pos[2], op[2]), c_rimg, c_nreal, c_fh,
+						c_fc, c_nfour);
+				ctfmm::M2L<float>(L, M, D, A.do_phi);
+				nev++;
+			}
+		}
+		float* Lo = A.L + int64_t(i) * kLStride;
+#pragma unroll
+		for (int q = 0; q < 50; q++) Lo[q] = L[q];
+	}
+	nev = wave_sum_u64(nev);
+	if (lane_id() == 0 && nev) atomicAdd(&A.cnt->stat[2], nev);
+}
+
+__global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	unsigned long long ncc = 0, ncp = 0;
+	if (i < A.nnodes) {
+		const DNode s = A.nodes[i];
+		float L[50];
+		float* Lo = A.L + int64_t(i) * kLStride;
+#pragma unroll
+		for (int q = 0; q < 50; q++) L[q] = Lo[q];
+		for (int sg = A.head[int64_t(kCC) * A.nnodes + i]; sg >= 0; sg = A.seg[sg].next) {
+			const Seg S = A.seg[sg];
+			for (int j = 0; j < S.len; j++) {
+				const int o = A.out_cc[S.begin + j];
+				float M[37];
+				uint32_t op[3];
+				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
+				float D[50];
+				ctfmm::greens<float>(D, fdist(s.pos[0], op[0]), fdist(s.pos[1], op[1]), fdist(s.pos[2], op[2]));
+				ctfmm::M2L<float>(L, M, D, A.do_phi);
+				ncc++;
+			}
+		}
+		for (int sg = A.head[int64_t(kCP) * A.nnodes + i]; sg >= 0; sg = A.seg[sg].next) {
+			const Seg S = A.seg[sg];
+			for (int j = 0; j < S.len; j++) {
+				const DNode o = A.nodes[A.out_cp[S.begin + j]];
+				for (int p = o.pbegin; p < o.pend; p++) {
+					float D[50];
+					ctfmm::greens<float>(D, fdist(s.pos[0], A.px[p]), fdist(s.pos[1], A.py[p]), fdist(s.pos[2], A.pz[p]));
+#pragma unroll
+					for (int q = 0; q < 50; q++) L[q] += D[q];
+					ncp++;
+				}
+			}
+		}
+#pragma unroll
+		for (int q = 0; q < 50; q++) Lo[q] = L[q];
+	}
+	ncc = wave_sum_u64(ncc);
+	ncp = wave_sum_u64(ncp);
+	if (lane_id() == 0) {
+		if (ncc) atomicAdd(&A.cnt->stat[1], ncc);
+		if (ncp) atomicAdd(&A.cnt->stat[3], ncp);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// L2L sweep, one launch per level
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_l2l(const DNode* __restrict__ nodes, float* __restrict__ L, int begin, int end, bool do_phi) {
+	const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= end) return;
+	const DNode s = nodes[i];
+	if (s.parent < 0) return;
+	const DNode p = nodes[s.parent];
+	const float* La = L + int64_t(s.parent) * kLStride;
+	float Lp[50];
+#pragma unroll
+	for (int q = 0; q < 50; q++) Lp[q] = La[q];
+	float Lb[50];
+	ctfmm::L2L<float>(Lb, Lp, fdist(s.pos[0], p.pos[0]), fdist(s.pos[1], p.pos[1]), fdist(s.pos[2], p.pos[2]), do_phi);
+	float* Lo = L + int64_t(i) * kLStride;
+#pragma unroll
+	for (int q = 0; q < 50; q++) Lo[q] += Lb[q];
+}
+
+// ---------------------------------------------------------------------------
+// leaf kernel: wave per leaf
+// ---------------------------------------------------------------------------
+
+struct LeafArgs {
+	const DNode* nodes;
+	const int32_t* leaf_ids;
+	int32_t nleaves;
+	const float* multi;
+	const float* L;
+	const int32_t* head;
+	const Seg* seg;
+	const int32_t* out_pc;
+	const int32_t* out_pp;
+	int32_t nnodes;
+	const uint32_t* px;
+	const uint32_t* py;
+	const uint32_t* pz;
+	float* vx;
+	float* vy;
+	float* vz;
+	int8_t* rung;
+	float* gx;
+	float* gy;
+	float* gz;
+	float* phi;
+	double* part;  // [nleaves * 8]
+	ct_kick_params prm;
+	Counters* cnt;
+};
+
+constexpr int kTile = 512;
+constexpr int kMaxGroups = 4;  // leaves up to 256 particles
+
+template<int G>
+__device__ __forceinline__ void pp_tile(const uint32_t* __restrict__ tx, const uint32_t* __restrict__ ty, const uint32_t* __restrict__ tz, int tn,
+		const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, float h2, float hinv, float h2inv, float h3inv, double* gx, double* gy,
+		double* gz, double* ph) {
+	float fx[G], fy[G], fz[G], pt[G];
+#pragma unroll
+	for (int g = 0; g < G; g++) {
+		fx[g] = fy[g] = fz[g] = pt[g] = 0.0f;
+	}
+	for (int t = 0; t < tn; t++) {
+		const uint32_t X = tx[t], Y = ty[t], Z = tz[t];
+#pragma unroll
+		for (int g = 0; g < G; g++) {
+			const float dx = float(int32_t(sx[g] - X));
+			const float dy = float(int32_t(sy[g] - Y));
+			const float dz = float(int32_t(sz[g] - Z));
+			const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+			float rinv = __builtin_amdgcn_rsqf(r2);
+			float r3 = rinv * rinv * rinv;
+			if (r2 <= h2) {
+				// Ferrers n=4 softening in q^2 = r^2/h^2 (cosmictiger/gravity.hpp:76-96)
+				const float q2 = r2 * h2inv;
+				float f = 315.0f / 128.0f;
+				f = __builtin_fmaf(f, q2, -385.0f / 32.0f);
+				f = __builtin_fmaf(f, q2, 1485.0f / 64.0f);
+				f = __builtin_fmaf(f, q2, -693.0f / 32.0f);
+				f = __builtin_fmaf(f, q2, 1155.0f / 128.0f);
+				float p = -63.0f / 256.0f;
+				p = __builtin_fmaf(p, q2, 385.0f / 256.0f);
+				p = __builtin_fmaf(p, q2, -495.0f / 128.0f);
+				p = __builtin_fmaf(p, q2, 693.0f / 128.0f);
+				p = __builtin_fmaf(p, q2, -1155.0f / 256.0f);
+				p = __builtin_fmaf(p, q2, 693.0f / 256.0f);
+				r3 = f * h3inv;
+				rinv = p * hinv;
+			}
+			fx[g] = __builtin_fmaf(dx, r3, fx[g]);
+			fy[g] = __builtin_fmaf(dy, r3, fy[g]);
+			fz[g] = __builtin_fmaf(dz, r3, fz[g]);
+			pt[g] -= rinv;
+		}
+	}
+	// integer units -> box units: force 2^64, potential 2^32
+#pragma unroll
+	for (int g = 0; g < G; g++) {
+		gx[g] -= double(fx[g]) * 18446744073709551616.0;
+		gy[g] -= double(fy[g]) * 18446744073709551616.0;
+		gz[g] -= double(fz[g]) * 18446744073709551616.0;
+		ph[g] += double(pt[g]) * 4294967296.0;
+	}
+}
+
+__global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
+	__shared__ uint32_t tx[kTile], ty[kTile], tz[kTile];
+	const int li = blockIdx.x;
+	if (li >= A.nleaves) return;
+	const int self = uniform(A.leaf_ids[li]);
+	const DNode s = A.nodes[self];
+	const int lane = threadIdx.x;
+	const int nsink = s.pend - s.pbegin;
+	double* part = A.part + int64_t(li) * 8;
+	if (nsink == 0) {
+		if (lane < 8) part[lane] = 0.0;
+		return;
+	}
+	const ct_kick_params& P = A.prm;
+	const int ng = min((nsink + 63) >> 6, kMaxGroups);
+	uint32_t sx[kMaxGroups], sy[kMaxGroups], sz[kMaxGroups];
+	double gx[kMaxGroups], gy[kMaxGroups], gz[kMaxGroups], ph[kMaxGroups];
+	const float hf = P.h;
+	const float hinv = 1.0f / hf;
+#pragma unroll
+	for (int g = 0; g < kMaxGroups; g++) {
+		int k = g * 64 + lane;
+		if (k >= nsink) k = 0;
+		const int pi = s.pbegin + k;
+		sx[g] = A.px[pi];
+		sy[g] = A.py[pi];
+		sz[g] = A.pz[pi];
+		gx[g] = gy[g] = gz[g] = 0.0;
+		ph[g] = double(kSelfPhi * hinv);
+	}
+	// ---- P2P in integer units (h in units of 2^-32)
+	const float hi_ = hf * 4294967296.0f;
+	const float h2 = hi_ * hi_;
+	const float ihinv = 1.0f / hi_;
+	const float ih2inv = ihinv * ihinv;
+	const float ih3inv = ih2inv * ihinv;
+	unsigned long long nsrc = 0;
+	int tn = 0;
+	auto flush = [&]() {
+		__syncthreads();
+		switch (ng) {
+		case 1:
+			pp_tile<1>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
+			break;
+		case 2:
+			pp_tile<2>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
+			break;
+		case 3:
+			pp_tile<3>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
+			break;
+		default:
+			pp_tile<4>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
+			break;
+		}
+		__syncthreads();
+		tn = 0;
+	};
+	for (int sg = uniform(A.head[int64_t(kPP) * A.nnodes + self]); sg >= 0; sg = uniform(A.seg[sg].next)) {
+		const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+		for (int j = 0; j < sl; j++) {
+			const int o = uniform(A.out_pp[sb + j]);
+			int pb = uniform(A.nodes[o].pbegin);
+			const int pe = uniform(A.nodes[o].pend);
+			nsrc += pe - pb;
+			while (pb < pe) {
+				const int m = min(kTile - tn, pe - pb);
+				for (int t = lane; t < m; t += 64) {
+					tx[tn + t] = A.px[pb + t];
+					ty[tn + t] = A.py[pb + t];
+					tz[tn + t] = A.pz[pb + t];
+				}
+				tn += m;
+				pb += m;
+				if (tn == kTile) flush();
+			}
+		}
+	}
+	if (tn) flush();
+	// ---- M2P (src/gravity.cu:157-211)
+	unsigned long long npc = 0;
+	for (int sg = uniform(A.head[int64_t(kPC) * A.nnodes + self]); sg >= 0; sg = uniform(A.seg[sg].next)) {
+		const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+		for (int j = 0; j < sl; j++) {
+			const int o = uniform(A.out_pc[sb + j]);
+			float M[37];
+			uint32_t op[3];
+			load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
+			npc++;
+			for (int g = 0; g < ng; g++) {
+				float D[50];
+				ctfmm::greens<float>(D, fdist(sx[g], op[0]), fdist(sy[g], op[1]), fdist(sz[g], op[2]));
+				float L4[4] = {0.f, 0.f, 0.f, 0.f};
+				ctfmm::M2L2<float>(L4, M, D, P.do_phi);
+				gx[g] -= double(1e6f * L4[1]);
+				gy[g] -= double(1e6f * L4[2]);
+				gz[g] -= double(1e6f * L4[3]);
+				ph[g] += double(1e3f * L4[0]);
+			}
+		}
+	}
+	// ---- L2P + kick (src/kick.cu:53-177)
+	float Ls[50];
+	{
+		const float* Lp = A.L + int64_t(self) * kLStride;
+#pragma unroll
+		for (int q = 0; q < 50; q++) Ls[q] = Lp[q];
+	}
+	double kin = 0, dkin = 0, xm = 0, ym = 0, zm = 0, nm = 0, pot = 0;
+	int maxr = 0;
+	const float sgn = P.sign * (P.top ? 1.f : -1.f);
+	for (int g = 0; g < ng; g++) {
+		const int k = g * 64 + lane;
+		if (k >= nsink) continue;
+		const int pi = s.pbegin + k;
+		float L4[4];
+		ctfmm::L2P<float>(L4, Ls, fdist(sx[g], s.pos[0]), fdist(sy[g], s.pos[1]), fdist(sz[g], s.pos[2]), P.do_phi);
+		float Fphi = float(ph[g]) + 1e3f * L4[0];
+		float Fx = float(gx[g]) - 1e6f * L4[1];
+		float Fy = float(gy[g]) - 1e6f * L4[2];
+		float Fz = float(gz[g]) - 1e6f * L4[3];
+		Fx *= P.GM;
+		Fy *= P.GM;
+		Fz *= P.GM;
+		Fphi *= P.GM;
+		float vx = A.vx[pi], vy = A.vy[pi], vz = A.vz[pi];
+		const float kin0 = 0.5f * (vx * vx + vy * vy + vz * vz);
+		float dt;
+		if (P.ascending) {
+			dt = 0.5f * c_rung_dt[P.min_rung] * P.t0;
+			if (!P.first_call) {
+				vx = __builtin_fmaf(sgn * Fx, dt, vx);
+				vy = __builtin_fmaf(sgn * Fy, dt, vy);
+				vz = __builtin_fmaf(sgn * Fz, dt, vz);
+			}
+		}
+		kin += 0.5 * double(vx * vx + vy * vy + vz * vz);
+		xm += vx;
+		ym += vy;
+		zm += vz;
+		nm += sqrtf(vx * vx + vy * vy + vz * vz);
+		if (P.save_force) {
+			A.gx[pi] = Fx;
+			A.gy[pi] = Fy;
+			A.gz[pi] = Fz;
+			A.phi[pi] = Fphi;
+		}
+		if (P.descending) {
+			const float g2 = Fx * Fx + Fy * Fy + Fz * Fz;
+			dt = fminf(P.eta * sqrtf(P.a * hf * rsqrtf(g2)), P.t0);
+			dt = fminf(P.max_dt, dt);
+			int r = A.rung[pi];
+			r = max(P.min_rung + int(int(ceilf(log2f(P.t0 / dt))) > P.min_rung), r - 1);
+			A.rung[pi] = int8_t(r);
+			maxr = max(maxr, r);
+			dt = 0.5f * c_rung_dt[P.min_rung] * P.t0;
+			vx = __builtin_fmaf(sgn * Fx, dt, vx);
+			vy = __builtin_fmaf(sgn * Fy, dt, vy);
+			vz = __builtin_fmaf(sgn * Fz, dt, vz);
+		}
+		const float kin1 = 0.5f * (vx * vx + vy * vy + vz * vz);
+		A.vx[pi] = vx;
+		A.vy[pi] = vy;
+		A.vz[pi] = vz;
+		pot += 0.5 * double(Fphi);
+		dkin += double(kin1) - double(kin0);
+	}
+	// wave reductions (FP64)
+	for (int o = 32; o > 0; o >>= 1) {
+		kin += __shfl_xor(kin, o);
+		dkin += __shfl_xor(dkin, o);
+		xm += __shfl_xor(xm, o);
+		ym += __shfl_xor(ym, o);
+		zm += __shfl_xor(zm, o);
+		nm += __shfl_xor(nm, o);
+		pot += __shfl_xor(pot, o);
+		maxr = max(maxr, __shfl_xor(maxr, o));
+	}
+	if (lane == 0) {
+		part[0] = double(maxr);
+		part[1] = pot;
+		part[2] = kin;
+		part[3] = dkin;
+		part[4] = xm;
+		part[5] = ym;
+		part[6] = zm;
+		part[7] = nm;
+		atomicAdd(&A.cnt->stat[0], (unsigned long long) nsrc * (unsigned long long) nsink);
+		if (npc) atomicAdd(&A.cnt->stat[4], npc * (unsigned long long) nsink);
+		atomicAdd(&A.cnt->stat[6], (unsigned long long) nsink);
+	}
+}
+
+// deterministic FP64 reduction of the per-leaf kick_return partials
+__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part, int n, double* __restrict__ out) {
+	__shared__ double sh[8][1024];
+	const int t = threadIdx.x;
+	double acc[8];
+	for (int q = 0; q < 8; q++) acc[q] = 0.0;
+	for (int i = t; i < n; i += 1024) {
+		acc[0] = fmax(acc[0], part[int64_t(i) * 8]);
+		for (int q = 1; q < 8; q++) acc[q] += part[int64_t(i) * 8 + q];
+	}
+	for (int q = 0; q < 8; q++) sh[q][t] = acc[q];
+	__syncthreads();
+	for (int w = 512; w > 0; w >>= 1) {
+		if (t < w) {
+			sh[0][t] = fmax(sh[0][t], sh[0][t + w]);
+			for (int q = 1; q < 8; q++) sh[q][t] += sh[q][t + w];
+		}
+		__syncthreads();
+	}
+	if (t < 8) out[t] = sh[t][0];
+}
+
+// ---------------------------------------------------------------------------
+// drift (src/drift.cu:36-154 without the lightcone branch)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_drift(uint32_t* __restrict__ x, uint32_t* __restrict__ y, uint32_t* __restrict__ z, float* __restrict__ vx,
+		float* __restrict__ vy, float* __restrict__ vz, const int8_t* __restrict__ rung, int64_t n, int r, double c1, double dragc, bool do_drag) {
+	const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	if (rung[i] != r) return;
+	uint32_t* X[3] = {x, y, z};
+	float* V[3] = {vx, vy, vz};
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const double x0 = double(X[d][i]) * (1.0 / 4294967296.0);
+		double x1 = __builtin_fma(double(V[d][i]), c1, x0);
+		x1 -= double(x1 >= 1.0);
+		x1 += double(x1 < 0.0);
+		X[d][i] = uint32_t(uint64_t(x1 * 4294967296.0));
+	}
+	if (do_drag) {
+#pragma unroll
+		for (int d = 0; d < 3; d++) V[d][i] = float(double(V[d][i]) * dragc);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// expansion kernels on the device, for checks
+// ---------------------------------------------------------------------------
+
+__global__ void k_fmm(int op, int64_t n, const double* __restrict__ in, double* __restrict__ out, int in_stride, int out_stride) {
+	const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const double* a = in + i * in_stride;
+	double* b = out + i * out_stride;
+	float X[3], A_[50], B_[50], C_[50];
+	switch (op) {
+	case 0:
+	case 1:
+	case 5: {
+		for (int q = 0; q < 3; q++) X[q] = float(a[q]);
+		if (op == 0) ctfmm::greens<float>(C_, X[0], X[1], X[2]);
+		if (op == 1) ctfmm::ewald_greens<float>(C_, X[0], X[1], X[2], c_rimg, c_nreal, c_fh, c_fc, c_nfour);
+		if (op == 5) ctfmm::P2M<float>(C_, X[0], X[1], X[2]);
+		const int no = op == 5 ? 37 : 50;
+		for (int q = 0; q < no; q++) b[q] = C_[q];
+		break;
+	}
+	case 2:
+	case 7: {
+		for (int q = 0; q < 37; q++) A_[q] = float(a[q]);
+		for (int q = 0; q < 50; q++) B_[q] = float(a[37 + q]);
+		for (int q = 0; q < 50; q++) C_[q] = 0.0f;
+		if (op == 2) {
+			ctfmm::M2L<float>(C_, A_, B_, true);
+			for (int q = 0; q < 50; q++) b[q] = C_[q];
+		} else {
+			ctfmm::M2L2<float>(C_, A_, B_, true);
+			for (int q = 0; q < 4; q++) b[q] = C_[q];
+		}
+		break;
+	}
+	case 3:
+	case 4: {
+		for (int q = 0; q < 50; q++) A_[q] = float(a[q]);
+		for (int q = 0; q < 3; q++) X[q] = float(a[50 + q]);
+		if (op == 3) {
+			ctfmm::L2L<float>(C_, A_, X[0], X[1], X[2], true);
+			for (int q = 0; q < 50; q++) b[q] = C_[q];
+		} else {
+			ctfmm::L2P<float>(C_, A_, X[0], X[1], X[2], true);
+			for (int q = 0; q < 4; q++) b[q] = C_[q];
+		}
+		break;
+	}
+	case 6: {
+		for (int q = 0; q < 37; q++) A_[q] = float(a[q]);
+		for (int q = 0; q < 3; q++) X[q] = float(a[37 + q]);
+		ctfmm::M2M<float>(C_, A_, X[0], X[1], X[2]);
+		for (int q = 0; q < 37; q++) b[q] = C_[q];
+		break;
+	}
+	default:
+		break;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+	g_err = msg;
+	return code;
+}
+
+#define CT_HIP(call)                                                                                                        \
+	do {                                                                                                                    \
+		hipError_t e_ = (call);                                                                                             \
+		if (e_ != hipSuccess) return fail(CT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));                  \
+	} while (0)
+
+struct EwaldHost {
+	std::vector<float> rimg, fh, fc;
+	int nreal = 0, nfour = 0;
+};
+
+// Ewald tables from the reference's rules (src/ewald_indices.cu:30-74):
+// real images n != 0 with "box distance" |max(|n|-1/2, 0)| < 2.6, Fourier
+// vectors 0 < |h|^2 <= 8; the Fourier part is folded to one of each +-h pair.
+EwaldHost make_ewald_tables() {
+	EwaldHost E;
+	const double a = 2.0;
+	std::vector<std::array<int, 3>> real;
+	for (int i = -4; i <= 4; i++)
+		for (int j = -4; j <= 4; j++)
+			for (int k = -4; k <= 4; k++) {
+				const double x = std::max(std::abs(i) - 0.5, 0.0);
+				const double y = std::max(std::abs(j) - 0.5, 0.0);
+				const double z = std::max(std::abs(k) - 0.5, 0.0);
+				if (std::sqrt(x * x + y * y + z * z) < 2.6 && (i || j || k)) real.push_back({i, j, k});
+			}
+	// far images first (smallest terms summed first)
+	std::stable_sort(real.begin(), real.end(), [](const std::array<int, 3>& p, const std::array<int, 3>& q) {
+		return p[0] * p[0] + p[1] * p[1] + p[2] * p[2] > q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+	});
+	for (auto& r : real) {
+		for (int d = 0; d < 3; d++) E.rimg.push_back(float(r[d]));
+	}
+	E.nreal = int(real.size());
+	std::vector<std::array<int, 3>> four;
+	for (int i = -3; i <= 3; i++)
+		for (int j = -3; j <= 3; j++)
+			for (int k = -3; k <= 3; k++) {
+				const int h2 = i * i + j * j + k * k;
+				if (h2 == 0 || h2 > 8) continue;
+				if (i < 0 || (i == 0 && j < 0) || (i == 0 && j == 0 && k < 0)) continue;
+				four.push_back({i, j, k});
+			}
+	std::stable_sort(four.begin(), four.end(), [](const std::array<int, 3>& p, const std::array<int, 3>& q) {
+		return p[0] * p[0] + p[1] * p[1] + p[2] * p[2] > q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+	});
+	E.nfour = int(four.size());
+	const double sgn4[4] = {1.0, -1.0, -1.0, 1.0};
+	for (auto& h : four) {
+		for (int d = 0; d < 3; d++) E.fh.push_back(float(h[d]));
+		const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+		const double c = std::exp(-M_PI * M_PI * h2 / (a * a)) / (M_PI * h2);
+		std::vector<float> row(50, 0.0f);
+		for (int l = 0; l < 7; l++)
+			for (int p = l; p >= 0; p--)
+				for (int q = l - p; q >= 0; q--) {
+					const int r = l - p - q;
+					if (!(r <= 1 || (p == 0 && q == 0 && r == 2))) continue;
+					const int idx = (p + q) * (p + q + 1) / 2 + q + 28 * (r == 1) + 49 * (r == 2);
+					const double hn = std::pow(double(h[0]), p) * std::pow(double(h[1]), q) * std::pow(double(h[2]), r);
+					row[idx] = float(-2.0 * c * std::pow(2.0 * M_PI, l) * hn * sgn4[l & 3]);
+				}
+		E.fc.insert(E.fc.end(), row.begin(), row.end());
+	}
+	return E;
+}
+
+struct DevState {
+	int device = -1;
+	bool ewald_ready = false;
+	std::mutex mtx;
+};
+DevState g_dev;
+
+int upload_ewald() {
+	const EwaldHost E = make_ewald_tables();
+	if (E.nreal > kNRealMax || E.nfour > kNFourMax) return fail(CT_E_ARG, "ewald tables too large");
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rimg), E.rimg.data(), E.rimg.size() * sizeof(float)));
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fh), E.fh.data(), E.fh.size() * sizeof(float)));
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fc), E.fc.data(), E.fc.size() * sizeof(float)));
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_nreal), &E.nreal, sizeof(int)));
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_nfour), &E.nfour, sizeof(int)));
+	float rdt[16];
+	for (int i = 0; i < 16; i++) rdt[i] = 1.0f / float(1 << i);
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rung_dt), rdt, sizeof(rdt)));
+	g_dev.ewald_ready = true;
+	return CT_OK;
+}
+
+template<class T>
+struct DBuf {
+	T* p = nullptr;
+	int64_t n = 0;
+	int grow(int64_t want) {
+		if (want <= n) return CT_OK;
+		if (p) (void) hipFree(p);
+		p = nullptr;
+		n = 0;
+		if (hipMalloc(&p, std::max<int64_t>(want, 1) * sizeof(T)) != hipSuccess) {
+			p = nullptr;
+			return fail(CT_E_OOM, "hipMalloc failed");
+		}
+		n = want;
+		return CT_OK;
+	}
+	void release() {
+		if (p) (void) hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+};
+
+}  // namespace ct
+
+using namespace ct;
+
+struct ct_box {
+	HostParticles hp;
+	HostTree tree;
+	bool have_tree = false;
+	bool host_stale = false;  // device particles are newer than the host copy
+	double theta = 0.7, hsoft = 0.0;
+	int bucket = 128;
+	// device particle state (tree order)
+	DBuf<uint32_t> x, y, z;
+	DBuf<float> vx, vy, vz, gx, gy, gz, phi;
+	DBuf<int8_t> rung;
+	// tree
+	DBuf<DNode> nodes;
+	DBuf<float> multi, L;
+	DBuf<int32_t> leaf_ids;
+	int64_t nnodes = 0, nleaves = 0;
+	// walk
+	DBuf<Act> act[2];
+	DBuf<int32_t> dp[2], ep[2];
+	DBuf<int32_t> out[kNumLists];
+	DBuf<Seg> seg;
+	DBuf<int32_t> head;
+	DBuf<Counters> cnt;
+	DBuf<double> part, red;
+	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
+	// diagnostics
+	hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+	double stats[12] = {0};
+	Counters last_counters;
+	bool have_lists = false;
+};
+
+namespace {
+
+int ensure_walk_buffers(ct_box* b) {
+	int rc;
+	for (int q = 0; q < 2; q++) {
+		if ((rc = b->act[q].grow(b->cap_act))) return rc;
+		if ((rc = b->dp[q].grow(b->cap_dp))) return rc;
+		if ((rc = b->ep[q].grow(b->cap_ep))) return rc;
+	}
+	for (int q = 0; q < kNumLists; q++) {
+		if ((rc = b->out[q].grow(b->cap_out[q]))) return rc;
+	}
+	if ((rc = b->seg.grow(b->cap_seg))) return rc;
+	if ((rc = b->head.grow(int64_t(kNumLists) * b->nnodes))) return rc;
+	if ((rc = b->cnt.grow(1))) return rc;
+	return CT_OK;
+}
+
+int upload_box(ct_box* b) {
+	const int64_t n = b->hp.n;
+	int rc;
+	if ((rc = b->x.grow(n)) || (rc = b->y.grow(n)) || (rc = b->z.grow(n)) || (rc = b->vx.grow(n)) || (rc = b->vy.grow(n)) ||
+			(rc = b->vz.grow(n)) || (rc = b->rung.grow(n)) || (rc = b->gx.grow(n)) || (rc = b->gy.grow(n)) || (rc = b->gz.grow(n)) ||
+			(rc = b->phi.grow(n)))
+		return rc;
+	if (n) {
+		CT_HIP(hipMemcpy(b->x.p, b->hp.x.data(), n * 4, hipMemcpyHostToDevice));
+		CT_HIP(hipMemcpy(b->y.p, b->hp.y.data(), n * 4, hipMemcpyHostToDevice));
+		CT_HIP(hipMemcpy(b->z.p, b->hp.z.data(), n * 4, hipMemcpyHostToDevice));
+		CT_HIP(hipMemcpy(b->vx.p, b->hp.vx.data(), n * 4, hipMemcpyHostToDevice));
+		CT_HIP(hipMemcpy(b->vy.p, b->hp.vy.data(), n * 4, hipMemcpyHostToDevice));
+		CT_HIP(hipMemcpy(b->vz.p, b->hp.vz.data(), n * 4, hipMemcpyHostToDevice));
+		CT_HIP(hipMemcpy(b->rung.p, b->hp.rung.data(), n, hipMemcpyHostToDevice));
+		CT_HIP(hipMemset(b->gx.p, 0, n * 4));
+		CT_HIP(hipMemset(b->gy.p, 0, n * 4));
+		CT_HIP(hipMemset(b->gz.p, 0, n * 4));
+		CT_HIP(hipMemset(b->phi.p, 0, n * 4));
+	}
+	return CT_OK;
+}
+
+int download_particles(ct_box* b) {
+	const int64_t n = b->hp.n;
+	if (!b->host_stale || n == 0) return CT_OK;
+	CT_HIP(hipDeviceSynchronize());
+	CT_HIP(hipMemcpy(b->hp.x.data(), b->x.p, n * 4, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.y.data(), b->y.p, n * 4, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.z.data(), b->z.p, n * 4, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.vx.data(), b->vx.p, n * 4, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.vy.data(), b->vy.p, n * 4, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.vz.data(), b->vz.p, n * 4, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.rung.data(), b->rung.p, n, hipMemcpyDeviceToHost));
+	b->host_stale = false;
+	return CT_OK;
+}
+
+int check_init() {
+	if (g_dev.device < 0) return fail(CT_E_STATE, "ct_init has not been called");
+	return CT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_init(int device) {
+	std::lock_guard<std::mutex> lk(g_dev.mtx);
+	int count = 0;
+	CT_HIP(hipGetDeviceCount(&count));
+	if (device < 0 || device >= count) return fail(CT_E_ARG, "ct_init: no such device");
+	CT_HIP(hipSetDevice(device));
+	g_dev.device = device;
+	return upload_ewald();
+}
+
+int ct_ewald_init(void) {
+	int rc = check_init();
+	if (rc) return rc;
+	return upload_ewald();
+}
+
+int ct_box_create(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, const float* vel, const int8_t* rung, ct_box_t* out) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!out || n < 0 || n > INT32_MAX || (n > 0 && (!x || !y || !z))) return fail(CT_E_ARG, "ct_box_create: bad arguments");
+	ct_box* b = new (std::nothrow) ct_box();
+	if (!b) return fail(CT_E_OOM, "ct_box_create: host allocation failed");
+	HostParticles& P = b->hp;
+	P.n = n;
+	P.x.assign(x, x + n);
+	P.y.assign(y, y + n);
+	P.z.assign(z, z + n);
+	P.vx.resize(n);
+	P.vy.resize(n);
+	P.vz.resize(n);
+	P.rung.resize(n);
+	P.id.resize(n);
+	for (int64_t i = 0; i < n; i++) {
+		P.vx[i] = vel ? vel[3 * i + 0] : 0.0f;
+		P.vy[i] = vel ? vel[3 * i + 1] : 0.0f;
+		P.vz[i] = vel ? vel[3 * i + 2] : 0.0f;
+		P.rung[i] = rung ? rung[i] : 0;
+		P.id[i] = int32_t(i);
+	}
+	for (int q = 0; q < 6; q++) {
+		if (hipEventCreate(&b->ev[q]) != hipSuccess) {
+			delete b;
+			return fail(CT_E_HIP, "hipEventCreate failed");
+		}
+	}
+	if ((rc = upload_box(b))) {
+		ct_box_destroy(b);
+		return rc;
+	}
+	*out = b;
+	return CT_OK;
+}
+
+int ct_box_destroy(ct_box_t b) {
+	if (!b) return CT_OK;
+	(void) hipDeviceSynchronize();
+	b->x.release();
+	b->y.release();
+	b->z.release();
+	b->vx.release();
+	b->vy.release();
+	b->vz.release();
+	b->gx.release();
+	b->gy.release();
+	b->gz.release();
+	b->phi.release();
+	b->rung.release();
+	b->nodes.release();
+	b->multi.release();
+	b->L.release();
+	b->leaf_ids.release();
+	for (int q = 0; q < 2; q++) {
+		b->act[q].release();
+		b->dp[q].release();
+		b->ep[q].release();
+	}
+	for (int q = 0; q < kNumLists; q++) b->out[q].release();
+	b->seg.release();
+	b->head.release();
+	b->cnt.release();
+	b->part.release();
+	b->red.release();
+	for (int q = 0; q < 6; q++)
+		if (b->ev[q]) (void) hipEventDestroy(b->ev[q]);
+	delete b;
+	return CT_OK;
+}
+
+int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!b || !(theta > 0.0) || bucket < 1 || bucket > 64 * kMaxGroups || hsoft < 0.0) return fail(CT_E_ARG, "ct_tree_build: bad arguments");
+	if ((rc = download_particles(b))) return rc;
+	TreeBuildParams prm;
+	prm.theta = theta;
+	prm.bucket = bucket;
+	prm.hsoft = hsoft;
+	std::string err;
+	HostTree T;
+	if ((rc = build_tree(b->hp, prm, T, &err))) return fail(rc, err);
+	b->tree = std::move(T);
+	b->theta = theta;
+	b->bucket = bucket;
+	b->hsoft = hsoft;
+	const int64_t nn = int64_t(b->tree.nodes.size());
+	b->nnodes = nn;
+	std::vector<DNode> dn(nn);
+	std::vector<float> mp(nn * kMultiStride, 0.0f);
+	std::vector<int32_t> leaves;
+	for (int64_t i = 0; i < nn; i++) {
+		const HostNode& h = b->tree.nodes[i];
+		DNode& d = dn[i];
+		for (int q = 0; q < 3; q++) {
+			d.pos[q] = h.pos[q];
+			d.bb[q] = h.box_begin[q];
+			d.be[q] = h.box_end[q];
+		}
+		d.radius = h.radius;
+		d.child[0] = h.child[0];
+		d.child[1] = h.child[1];
+		d.pbegin = h.pbegin;
+		d.pend = h.pend;
+		d.flags = (h.leaf ? 1 : 0) | (h.depth << 1);
+		d.parent = h.parent;
+		std::memcpy(&mp[i * kMultiStride], h.multi, 37 * sizeof(float));
+		std::memcpy(&mp[i * kMultiStride + 37], h.pos, 3 * sizeof(uint32_t));
+		if (h.leaf) leaves.push_back(int32_t(i));
+	}
+	b->nleaves = int64_t(leaves.size());
+	if ((rc = b->nodes.grow(nn)) || (rc = b->multi.grow(nn * kMultiStride)) || (rc = b->L.grow(nn * kLStride)) ||
+			(rc = b->leaf_ids.grow(std::max<int64_t>(1, b->nleaves))) || (rc = b->part.grow(std::max<int64_t>(1, b->nleaves) * 8)) ||
+			(rc = b->red.grow(8)))
+		return rc;
+	CT_HIP(hipMemcpy(b->nodes.p, dn.data(), nn * sizeof(DNode), hipMemcpyHostToDevice));
+	CT_HIP(hipMemcpy(b->multi.p, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice));
+	if (b->nleaves) CT_HIP(hipMemcpy(b->leaf_ids.p, leaves.data(), leaves.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+	if ((rc = upload_box(b))) return rc;
+	b->have_tree = true;
+	b->have_lists = false;
+	// initial list capacities; grown from the counters on overflow
+	if (b->cap_act == 0) {
+		b->cap_act = std::max<int64_t>(4 * nn, 1024);
+		b->cap_dp = std::max<int64_t>(64 * nn, 1 << 16);
+		b->cap_ep = std::max<int64_t>(16 * nn, 1 << 14);
+		b->cap_out[kEwald] = std::max<int64_t>(8 * nn, 1 << 14);
+		b->cap_out[kCC] = std::max<int64_t>(64 * nn, 1 << 16);
+		b->cap_out[kCP] = std::max<int64_t>(4 * nn, 1 << 12);
+		b->cap_out[kPC] = std::max<int64_t>(4 * nn, 1 << 12);
+		b->cap_out[kPP] = std::max<int64_t>(32 * nn, 1 << 16);
+		b->cap_seg = std::max<int64_t>(8 * nn, 1 << 14);
+	}
+	return CT_OK;
+}
+
+static int run_walk(ct_box* b, hipStream_t st, const ct_kick_params* p) {
+	const int iters = b->tree.max_depth + 3;
+	if (iters > kMaxIter) return fail(CT_E_TREE, "tree too deep for the walk");
+	for (int attempt = 0; attempt < 12; attempt++) {
+		int rc = ensure_walk_buffers(b);
+		if (rc) return rc;
+		InitArgs ia;
+		ia.head = b->head.p;
+		ia.nhead = int64_t(kNumLists) * b->nnodes;
+		ia.cnt = b->cnt.p;
+		ia.act0 = b->act[0].p;
+		ia.dp0 = b->dp[0].p;
+		ia.ep0 = b->ep[0].p;
+		ia.root = 0;
+		hipLaunchKernelGGL(k_walk_init, dim3(256), dim3(256), 0, st, ia);
+		hipLaunchKernelGGL(k_walk_seed_counts, dim3(1), dim3(64), 0, st, b->cnt.p);
+		WalkArgs A;
+		A.nodes = b->nodes.p;
+		A.nnodes = int32_t(b->nnodes);
+		for (int q = 0; q < 2; q++) {
+			A.act[q] = b->act[q].p;
+			A.dp[q] = b->dp[q].p;
+			A.ep[q] = b->ep[q].p;
+		}
+		A.cap_act = b->cap_act;
+		A.cap_dp = b->cap_dp;
+		A.cap_ep = b->cap_ep;
+		for (int q = 0; q < kNumLists; q++) {
+			A.out[q] = b->out[q].p;
+			A.cap_out[q] = b->cap_out[q];
+		}
+		A.seg = b->seg.p;
+		A.cap_seg = b->cap_seg;
+		A.head = b->head.p;
+		A.cnt = b->cnt.p;
+		A.theta_inv = 1.0f / p->theta;
+		A.h = p->h;
+		const int grid = 2048;
+		for (int k = 0; k < iters; k++) {
+			hipLaunchKernelGGL(k_walk, dim3(grid), dim3(256), 0, st, A, k);
+		}
+		Counters C;
+		CT_HIP(hipMemcpyAsync(&C, b->cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+		CT_HIP(hipStreamSynchronize(st));
+		if (C.walk_error) return fail(CT_E_WALK, "walk: a leaf failed the Ewald acceptance test");
+		if (C.act[iters] != 0 && !C.overflow) return fail(CT_E_WALK, "walk did not terminate within the tree depth");
+		if (!C.overflow) {
+			b->last_counters = C;
+			return CT_OK;
+		}
+		auto bump = [](int64_t& cap, int64_t need) { cap = std::max<int64_t>(2 * cap, need + need / 4 + 1024); };
+		int maxa = 0, maxd = 0, maxe = 0;
+		for (int k = 0; k <= iters; k++) {
+			maxa = std::max(maxa, C.act[k]);
+			maxd = std::max(maxd, C.dp[k]);
+			maxe = std::max(maxe, C.ep[k]);
+		}
+		if (C.overflow & (1 << 5)) bump(b->cap_act, maxa);
+		if (C.overflow & (1 << 6)) bump(b->cap_dp, maxd);
+		if (C.overflow & (1 << 7)) bump(b->cap_ep, maxe);
+		for (int q = 0; q < kNumLists; q++)
+			if (C.overflow & (1 << q)) bump(b->cap_out[q], C.out[q]);
+		if (C.overflow & (1 << 8)) bump(b->cap_seg, C.seg);
+	}
+	return fail(CT_E_ARENA, "walk: list buffers kept overflowing");
+}
+
+int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stream) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!b || !p) return fail(CT_E_ARG, "ct_kick: bad arguments");
+	if (!b->have_tree) return fail(CT_E_STATE, "ct_kick: no tree (call ct_tree_build)");
+	if (p->min_rung < 0 || p->min_rung >= 16 || !(p->theta > 0.0f) || !(p->h > 0.0f)) return fail(CT_E_ARG, "ct_kick: bad params");
+	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+	CT_HIP(hipEventRecord(b->ev[0], st));
+	if ((rc = run_walk(b, st, p))) return rc;
+	CT_HIP(hipEventRecord(b->ev[1], st));
+	CellArgs ca;
+	ca.nodes = b->nodes.p;
+	ca.multi = b->multi.p;
+	ca.L = b->L.p;
+	ca.head = b->head.p;
+	ca.seg = b->seg.p;
+	ca.out_e = b->out[kEwald].p;
+	ca.out_cc = b->out[kCC].p;
+	ca.out_cp = b->out[kCP].p;
+	ca.px = b->x.p;
+	ca.py = b->y.p;
+	ca.pz = b->z.p;
+	ca.nnodes = int32_t(b->nnodes);
+	ca.do_phi = p->do_phi;
+	ca.cnt = b->cnt.p;
+	const int nb = int((b->nnodes + 63) / 64);
+	hipLaunchKernelGGL(k_cells_ewald, dim3(nb), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL(k_cells_direct, dim3(nb), dim3(64), 0, st, ca);
+	CT_HIP(hipEventRecord(b->ev[2], st));
+	for (int d = 1; d <= b->tree.max_depth; d++) {
+		const int lb = b->tree.level_off[d], le = b->tree.level_off[d + 1];
+		if (le > lb) hipLaunchKernelGGL(k_l2l, dim3((le - lb + 63) / 64), dim3(64), 0, st, b->nodes.p, b->L.p, lb, le, p->do_phi);
+	}
+	CT_HIP(hipEventRecord(b->ev[3], st));
+	LeafArgs la;
+	la.nodes = b->nodes.p;
+	la.leaf_ids = b->leaf_ids.p;
+	la.nleaves = int32_t(b->nleaves);
+	la.multi = b->multi.p;
+	la.L = b->L.p;
+	la.head = b->head.p;
+	la.seg = b->seg.p;
+	la.out_pc = b->out[kPC].p;
+	la.out_pp = b->out[kPP].p;
+	la.nnodes = int32_t(b->nnodes);
+	la.px = b->x.p;
+	la.py = b->y.p;
+	la.pz = b->z.p;
+	la.vx = b->vx.p;
+	la.vy = b->vy.p;
+	la.vz = b->vz.p;
+	la.rung = b->rung.p;
+	la.gx = b->gx.p;
+	la.gy = b->gy.p;
+	la.gz = b->gz.p;
+	la.phi = b->phi.p;
+	la.part = b->part.p;
+	la.prm = *p;
+	la.cnt = b->cnt.p;
+	if (b->nleaves) hipLaunchKernelGGL(k_leaf, dim3(b->nleaves), dim3(64), 0, st, la);
+	CT_HIP(hipEventRecord(b->ev[4], st));
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, b->part.p, int(b->nleaves), b->red.p);
+	double red[8];
+	CT_HIP(hipMemcpyAsync(red, b->red.p, sizeof(red), hipMemcpyDeviceToHost, st));
+	CT_HIP(hipEventRecord(b->ev[5], st));
+	Counters C;
+	CT_HIP(hipMemcpyAsync(&C, b->cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+	CT_HIP(hipStreamSynchronize(st));
+	CT_HIP(hipGetLastError());
+	b->host_stale = true;
+	b->have_lists = true;
+	b->last_counters = C;
+	if (ret) {
+		ret->max_rung = int(red[0]);
+		ret->pot = red[1];
+		ret->kin = red[2];
+		ret->dkin = red[3];
+		ret->xmom = red[4];
+		ret->ymom = red[5];
+		ret->zmom = red[6];
+		ret->nmom = red[7];
+	}
+	float ms;
+	for (int q = 0; q < 4; q++) {
+		CT_HIP(hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]));
+		b->stats[q] = ms;
+	}
+	CT_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[4]));
+	b->stats[4] = ms;
+	b->stats[5] = double(C.stat[0]);
+	b->stats[6] = double(C.stat[1]);
+	b->stats[7] = double(C.stat[2]);
+	b->stats[8] = double(C.stat[3]);
+	b->stats[9] = double(C.stat[4]);
+	b->stats[10] = double(b->nnodes);
+	b->stats[11] = double(C.stat[6]);
+	return CT_OK;
+}
+
+int ct_drift(ct_box_t b, int rung, float a, float dt, double drag, void* stream) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!b || !(a > 0.0f)) return fail(CT_E_ARG, "ct_drift: bad arguments");
+	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+	const int64_t n = b->hp.n;
+	if (n == 0) return CT_OK;
+	const double c1 = double(dt) / double(a);
+	const double dragc = 1.0 / (1.0 + drag * double(dt));
+	hipLaunchKernelGGL(k_drift, dim3((n + 255) / 256), dim3(256), 0, st, b->x.p, b->y.p, b->z.p, b->vx.p, b->vy.p, b->vz.p, b->rung.p, n, rung,
+			c1, dragc, drag != 0.0);
+	CT_HIP(hipGetLastError());
+	CT_HIP(hipStreamSynchronize(st));
+	b->host_stale = true;
+	return CT_OK;
+}
+
+int ct_box_size(ct_box_t b, int64_t* np, int64_t* nn) {
+	if (!b) return fail(CT_E_ARG, "ct_box_size: null box");
+	if (np) *np = b->hp.n;
+	if (nn) *nn = b->nnodes;
+	return CT_OK;
+}
+
+int ct_box_read(ct_box_t b, uint32_t* x, uint32_t* y, uint32_t* z, float* vel, int8_t* rung, float* gx, float* gy, float* gz, float* phi, int32_t* id) {
+	if (!b) return fail(CT_E_ARG, "ct_box_read: null box");
+	const int64_t n = b->hp.n;
+	if (n == 0) return CT_OK;
+	CT_HIP(hipDeviceSynchronize());
+	if (x) CT_HIP(hipMemcpy(x, b->x.p, n * 4, hipMemcpyDeviceToHost));
+	if (y) CT_HIP(hipMemcpy(y, b->y.p, n * 4, hipMemcpyDeviceToHost));
+	if (z) CT_HIP(hipMemcpy(z, b->z.p, n * 4, hipMemcpyDeviceToHost));
+	if (vel) {
+		std::vector<float> t(n);
+		float* comp[3] = {b->vx.p, b->vy.p, b->vz.p};
+		for (int d = 0; d < 3; d++) {
+			CT_HIP(hipMemcpy(t.data(), comp[d], n * 4, hipMemcpyDeviceToHost));
+			for (int64_t i = 0; i < n; i++) vel[3 * i + d] = t[i];
+		}
+	}
+	if (rung) CT_HIP(hipMemcpy(rung, b->rung.p, n, hipMemcpyDeviceToHost));
+	if (gx) CT_HIP(hipMemcpy(gx, b->gx.p, n * 4, hipMemcpyDeviceToHost));
+	if (gy) CT_HIP(hipMemcpy(gy, b->gy.p, n * 4, hipMemcpyDeviceToHost));
+	if (gz) CT_HIP(hipMemcpy(gz, b->gz.p, n * 4, hipMemcpyDeviceToHost));
+	if (phi) CT_HIP(hipMemcpy(phi, b->phi.p, n * 4, hipMemcpyDeviceToHost));
+	if (id) std::memcpy(id, b->hp.id.data(), n * sizeof(int32_t));
+	return CT_OK;
+}
+
+static void host_node_to_abi(const HostNode& h, ct_node& o) {
+	for (int q = 0; q < 3; q++) {
+		o.box_begin[q] = h.box_begin[q];
+		o.box_end[q] = h.box_end[q];
+		o.pos[q] = h.pos[q];
+	}
+	o.child[0] = h.child[0];
+	o.child[1] = h.child[1];
+	o.parent = h.parent;
+	o.pbegin = h.pbegin;
+	o.pend = h.pend;
+	o.radius = h.radius;
+	o.leaf = h.leaf;
+	o.depth = h.depth;
+	std::memcpy(o.multi, h.multi, sizeof(o.multi));
+}
+
+int ct_tree_build_host(int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, int32_t* perm, double theta, int bucket, double hsoft,
+		ct_node* nodes, int64_t node_cap, int64_t* nnodes) {
+	if (n < 0 || n > INT32_MAX || (n > 0 && (!x || !y || !z)) || !(theta > 0.0) || bucket < 1 || bucket > 64 * kMaxGroups || !nnodes)
+		return fail(CT_E_ARG, "ct_tree_build_host: bad arguments");
+	HostParticles P;
+	P.n = n;
+	P.x.assign(x, x + n);
+	P.y.assign(y, y + n);
+	P.z.assign(z, z + n);
+	P.vx.assign(n, 0.0f);
+	P.vy.assign(n, 0.0f);
+	P.vz.assign(n, 0.0f);
+	P.rung.assign(n, 0);
+	P.id.resize(n);
+	for (int64_t i = 0; i < n; i++) P.id[i] = int32_t(i);
+	TreeBuildParams prm;
+	prm.theta = theta;
+	prm.bucket = bucket;
+	prm.hsoft = hsoft;
+	HostTree T;
+	std::string err;
+	int rc = build_tree(P, prm, T, &err);
+	if (rc) return fail(rc, err);
+	*nnodes = int64_t(T.nodes.size());
+	if (!nodes || node_cap < *nnodes) return fail(CT_E_ARG, "ct_tree_build_host: node capacity too small");
+	std::memcpy(x, P.x.data(), n * 4);
+	std::memcpy(y, P.y.data(), n * 4);
+	std::memcpy(z, P.z.data(), n * 4);
+	if (perm) std::memcpy(perm, P.id.data(), n * 4);
+	for (size_t i = 0; i < T.nodes.size(); i++) host_node_to_abi(T.nodes[i], nodes[i]);
+	return CT_OK;
+}
+
+int ct_tree_read(ct_box_t b, ct_node* nodes, int64_t cap) {
+	if (!b || !nodes) return fail(CT_E_ARG, "ct_tree_read: bad arguments");
+	if (!b->have_tree) return fail(CT_E_STATE, "ct_tree_read: no tree");
+	if (cap < b->nnodes) return fail(CT_E_ARG, "ct_tree_read: capacity too small");
+	for (int64_t i = 0; i < b->nnodes; i++) host_node_to_abi(b->tree.nodes[i], nodes[i]);
+	return CT_OK;
+}
+
+int ct_lists_read(ct_box_t b, int kind, int64_t* off, int32_t* idx, int64_t idx_cap) {
+	if (!b || kind < 0 || kind >= kNumLists || !off) return fail(CT_E_ARG, "ct_lists_read: bad arguments");
+	if (!b->have_lists) return fail(CT_E_STATE, "ct_lists_read: no kick has run");
+	const int64_t nn = b->nnodes;
+	std::vector<int32_t> head(nn);
+	const Counters& C = b->last_counters;
+	std::vector<Seg> seg(std::max(1, C.seg));
+	std::vector<int32_t> out(std::max(1, C.out[kind]));
+	CT_HIP(hipDeviceSynchronize());
+	CT_HIP(hipMemcpy(head.data(), b->head.p + int64_t(kind) * nn, nn * 4, hipMemcpyDeviceToHost));
+	if (C.seg) CT_HIP(hipMemcpy(seg.data(), b->seg.p, C.seg * sizeof(Seg), hipMemcpyDeviceToHost));
+	if (C.out[kind]) CT_HIP(hipMemcpy(out.data(), b->out[kind].p, int64_t(C.out[kind]) * 4, hipMemcpyDeviceToHost));
+	int64_t tot = 0;
+	for (int64_t i = 0; i < nn; i++) {
+		off[i] = tot;
+		// segments are chained newest first; emit oldest first
+		std::vector<int32_t> chain;
+		for (int s = head[i]; s >= 0; s = seg[s].next) chain.push_back(s);
+		for (auto it = chain.rbegin(); it != chain.rend(); ++it) {
+			const Seg& sg = seg[*it];
+			for (int j = 0; j < sg.len; j++) {
+				if (idx) {
+					if (tot >= idx_cap) return fail(CT_E_ARG, "ct_lists_read: capacity too small");
+					idx[tot] = out[sg.begin + j];
+				}
+				tot++;
+			}
+		}
+	}
+	off[nn] = tot;
+	return CT_OK;
+}
+
+int ct_kick_stats(ct_box_t b, double* out, int n) {
+	if (!b || !out) return fail(CT_E_ARG, "ct_kick_stats: bad arguments");
+	for (int q = 0; q < n && q < 12; q++) out[q] = b->stats[q];
+	return CT_OK;
+}
+
+int ct_fmm_host(int op, const double* in, double* out) {
+	if (!in || !out) return fail(CT_E_ARG, "ct_fmm_host: bad arguments");
+	switch (op) {
+	case 0:
+		ctfmm::greens<double>(out, in[0], in[1], in[2]);
+		return CT_OK;
+	case 1: {
+		const EwaldHost E = make_ewald_tables();
+		ctfmm::ewald_greens<double>(out, in[0], in[1], in[2], E.rimg.data(), E.nreal, E.fh.data(), E.fc.data(), E.nfour);
+		return CT_OK;
+	}
+	case 2:
+		for (int q = 0; q < 50; q++) out[q] = 0.0;
+		ctfmm::M2L<double>(out, in, in + 37, true);
+		return CT_OK;
+	case 3:
+		ctfmm::L2L<double>(out, in, in[50], in[51], in[52], true);
+		return CT_OK;
+	case 4:
+		ctfmm::L2P<double>(out, in, in[50], in[51], in[52], true);
+		return CT_OK;
+	case 5:
+		ctfmm::P2M<double>(out, in[0], in[1], in[2]);
+		return CT_OK;
+	case 6:
+		ctfmm::M2M<double>(out, in, in[37], in[38], in[39]);
+		return CT_OK;
+	case 7:
+		for (int q = 0; q < 4; q++) out[q] = 0.0;
+		ctfmm::M2L2<double>(out, in, in + 37, true);
+		return CT_OK;
+	default:
+		return fail(CT_E_ARG, "ct_fmm_host: unknown op");
+	}
+}
+
+int ct_fmm_device(int op, int64_t n, const double* in, double* out) {
+	int rc = check_init();
+	if (rc) return rc;
+	static const int in_stride[8] = {3, 3, 87, 53, 53, 3, 40, 87};
+	static const int out_stride[8] = {50, 50, 50, 50, 4, 37, 37, 4};
+	if (op < 0 || op > 7 || n < 0 || !in || !out) return fail(CT_E_ARG, "ct_fmm_device: bad arguments");
+	if (n == 0) return CT_OK;
+	double *din = nullptr, *dout = nullptr;
+	CT_HIP(hipMalloc(&din, n * in_stride[op] * sizeof(double)));
+	if (hipMalloc(&dout, n * out_stride[op] * sizeof(double)) != hipSuccess) {
+		(void) hipFree(din);
+		return fail(CT_E_OOM, "ct_fmm_device: hipMalloc failed");
+	}
+	(void) hipMemcpy(din, in, n * in_stride[op] * sizeof(double), hipMemcpyHostToDevice);
+	hipLaunchKernelGGL(k_fmm, dim3((n + 63) / 64), dim3(64), 0, 0, op, n, din, dout, in_stride[op], out_stride[op]);
+	hipError_t e = hipMemcpy(out, dout, n * out_stride[op] * sizeof(double), hipMemcpyDeviceToHost);
+	(void) hipFree(din);
+	(void) hipFree(dout);
+	if (e != hipSuccess) return fail(CT_E_HIP, std::string("ct_fmm_device: ") + hipGetErrorString(e));
+	return CT_OK;
+}
+
+const char* ct_last_error(void) {
+	return g_err.c_str();
+}
+
+}  // extern "C"

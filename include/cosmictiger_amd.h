@@ -1,0 +1,171 @@
+/*
+ * C-ABI of the MI355X-native CosmicTiger gravity kick (libcosmictiger_amd.so).
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every entry point
+ * returns CT_OK (0) or a negative CT_E_* code; ct_last_error() gives the
+ * message of the last failure on the calling thread.  No exception crosses
+ * this boundary.  A box handle may be used by one host thread at a time;
+ * different boxes are independent.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef COSMICTIGER_AMD_H
+#define COSMICTIGER_AMD_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CT_OK 0
+#define CT_E_ARG (-1)   /* invalid argument */
+#define CT_E_OOM (-2)   /* device or host allocation failed */
+#define CT_E_HIP (-3)   /* HIP runtime error */
+#define CT_E_ARENA (-4) /* interaction-list arena overflow that retries could not absorb */
+#define CT_E_STATE (-5) /* call out of order (e.g. kick before tree build) */
+#define CT_E_TREE (-6)  /* tree build failed (depth limit) */
+#define CT_E_WALK (-7)  /* walk invariant violated (a leaf failed the Ewald test) */
+
+/* Mirror of kick_params, same fields in the same order
+ * (cosmictiger/kick.hpp:138-157). */
+typedef struct ct_kick_params {
+	float sign;
+	int min_level;
+	int min_rung;
+	float a;
+	float t0;
+	float theta;
+	float h;
+	float eta;
+	float GM;
+	bool vel_init;
+	bool save_force;
+	bool first_call;
+	bool gpu;
+	float node_load;
+	float max_dt;
+	bool ascending;
+	bool descending;
+	bool top;
+	bool do_phi;
+} ct_kick_params;
+
+/* Mirror of kick_return (cosmictiger/kick.hpp:92-100). */
+typedef struct ct_kick_return {
+	int max_rung;
+	double pot;
+	double kin;
+	double dkin;
+	double xmom;
+	double ymom;
+	double zmom;
+	double nmom;
+} ct_kick_return;
+
+/* Node as read back by ct_tree_read (the fields of tree_node,
+ * cosmictiger/tree.hpp:98-169, with the multipole of multi_pos inline). */
+typedef struct ct_node {
+	int32_t box_begin[3]; /* fixed<int,29> raw: units of 2^-29 */
+	int32_t box_end[3];
+	int32_t child[2]; /* -1 for leaves */
+	int32_t parent;   /* -1 for the root */
+	int32_t pbegin;   /* part_range [pbegin, pend) */
+	int32_t pend;
+	uint32_t pos[3]; /* fixed32 expansion centre */
+	float radius;
+	int32_t leaf;
+	int32_t depth;
+	float multi[37]; /* multipole<float> (trless order 6, scaled frame) */
+} ct_node;
+
+typedef struct ct_box* ct_box_t;
+
+/* Select the device and build the Ewald tables.  Replaces cuda_init /
+ * cuda_set_device (cosmictiger/cuda.hpp:124-130) and ewald_const::init()
+ * (cosmictiger/ewald_indices.hpp:32-41, src/ewald_indices.cu:30-114). */
+int ct_init(int device);
+
+/* Rebuild the Ewald tables (ewald_const::init_gpu, src/ewald_indices.cu:30). */
+int ct_ewald_init(void);
+
+/* Create a resident box from host SoA positions (fixed32), AoS velocities
+ * float[3] and rungs.  The library owns the device copies.  Replaces the
+ * particle store setup (particles_resize / the globals particles_x/v/r,
+ * cosmictiger/particles.hpp:159-163, src/particles.cpp:774-839). */
+int ct_box_create(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, const float* vel, const int8_t* rung,
+		ct_box_t* out);
+
+/* Free a box (particles_destroy / tree_destroy). */
+int ct_box_destroy(ct_box_t box);
+
+/* Build the tree over the box with opening angle theta, leaf bucket size and
+ * softening hsoft; permutes the particles as the reference's partition does.
+ * Replaces tree_create (cosmictiger/tree.hpp:207, src/tree.cpp:273). */
+int ct_tree_build(ct_box_t box, double theta, int bucket, double hsoft);
+
+/* Host-only tree build over caller arrays (no device needed): the same build
+ * as ct_tree_build.  Permutes x, y, z in place; perm[i] receives the original
+ * index of the particle now at i.  Writes up to node_cap nodes (breadth-first,
+ * root = 0) and the node count to *nnodes; returns CT_E_ARG with *nnodes set
+ * when node_cap is too small.  Replaces tree_create (src/tree.cpp:273). */
+int ct_tree_build_host(int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, int32_t* perm, double theta, int bucket, double hsoft,
+		ct_node* nodes, int64_t node_cap, int64_t* nnodes);
+
+/* One kick of every particle of the box from the root, i.e. the reference's
+ * kick(params, L=0, pos=0, root, {root}, {root}, nullptr)
+ * (cosmictiger/kick.hpp:199, src/kick.cpp:150; device part src/kick.cu:508
+ * cuda_execute_kicks).  stream: a hipStream_t or NULL for the default stream.
+ * The call is synchronous with respect to the returned kick_return. */
+int ct_kick(ct_box_t box, const ct_kick_params* params, ct_kick_return* ret, void* stream);
+
+/* Drift particles of rung `rung` by dt at scale factor a (float arguments as
+ * the reference narrows them), with optional drag.  Replaces
+ * cuda_drift(char, float a, float dt, ...) (cosmictiger/drift.hpp:54,
+ * src/drift.cu:156) as called by drift() (src/drift.cpp:33). */
+int ct_drift(ct_box_t box, int rung, float a, float dt, double drag, void* stream);
+
+/* Number of particles and nodes of a box. */
+int ct_box_size(ct_box_t box, int64_t* nparts, int64_t* nnodes);
+
+/* Copy the box back to the host in the current (tree) order; any pointer may
+ * be NULL.  id[i] is the index the particle had in ct_box_create.  Replaces
+ * the particle accessors (cosmictiger/particles.hpp:221-306). */
+int ct_box_read(ct_box_t box, uint32_t* x, uint32_t* y, uint32_t* z, float* vel, int8_t* rung, float* gx, float* gy, float* gz,
+		float* phi, int32_t* id);
+
+/* Copy the node table (breadth-first order, root = 0) to the host.
+ * Replaces tree_get_node / tree_data (cosmictiger/tree.hpp:213,224). */
+int ct_tree_read(ct_box_t box, ct_node* nodes, int64_t cap);
+
+/* Interaction lists of the last kick, CSR by node, for the list kinds
+ * 0 = Ewald CC, 1 = direct CC, 2 = CP, 3 = PC, 4 = PP (the lists of
+ * src/kick.cu:269-423).  off has nnodes+1 entries; idx capacity idx_cap. */
+int ct_lists_read(ct_box_t box, int kind, int64_t* off, int32_t* idx, int64_t idx_cap);
+
+/* Per-phase device times (ms) of the last kick, measured with HIP events on
+ * the kick's stream: [0] walk, [1] cell interactions (M2L, Ewald, P2L),
+ * [2] L2L sweep, [3] leaf kernel (P2P, M2P, L2P, kick), [4] whole kick.
+ * Counters: [5] PP pairs, [6] direct CC, [7] Ewald CC, [8] CP particles,
+ * [9] PC sink-multipoles, [10] node visits, [11] sinks. */
+int ct_kick_stats(ct_box_t box, double* out, int n);
+
+/* Host evaluation of the expansion kernels in double, for checks of the
+ * layout: op 0 greens (in X[3] -> out D[50]), 1 ewald_greens (X -> D),
+ * 2 M2L (in M[37] D[50] -> L[50]), 3 L2L (La[50] X[3] -> Lb[50]),
+ * 4 L2P (La[50] X[3] -> L[4]), 5 P2M (X -> M[37]), 6 M2M (Ma[37] X[3] -> Mb[37]),
+ * 7 M2L order 2 (M[37] D[50] -> L[4]).  These are the templates of
+ * cosmictiger/fmm_kernels.hpp instantiated for double. */
+int ct_fmm_host(int op, const double* in, double* out);
+
+/* The same kernels on the device in float, for n independent inputs
+ * (in: n * in_stride doubles, out: n * out_stride doubles, host memory). */
+int ct_fmm_device(int op, int64_t n, const double* in, double* out);
+
+const char* ct_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

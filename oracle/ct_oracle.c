@@ -670,11 +670,12 @@ typedef struct {
 } build_ret;
 
 static int node_alloc(oc_tree* t) {
-	if (t->nnodes >= t->cap) {
-		t->cap = t->cap ? 2 * t->cap : 1024;
-		t->nodes = (oc_node*) realloc(t->nodes, t->cap * sizeof(oc_node));
+	const int64_t i = __atomic_fetch_add(&t->nnodes, 1, __ATOMIC_RELAXED);
+	if (i >= t->cap) {
+		t->error = 2;
+		return -1;
 	}
-	return (int) t->nnodes++;
+	return (int) i;
 }
 
 /* two-pointer partition on x[dim] >= fixed32(xm) (src/particles.cpp:979-1018) */
@@ -714,6 +715,11 @@ static build_ret build_node(build_ctx* c, int64_t b, int64_t e, const double box
 		return rc;
 	}
 	const int index = node_alloc(t);
+	if (index < 0) {
+		memset(&rc, 0, sizeof(rc));
+		rc.id = -1;
+		return rc;
+	}
 	const int64_t nparts = e - b;
 	float box_r = 0.0f;
 	for (int d = 0; d < 3; d++) {
@@ -746,8 +752,17 @@ static build_ret build_node(build_ctx* c, int64_t b, int64_t e, const double box
 			le[d] = re[d] = box_e[d];
 		}
 		le[xdim] = rb[xdim] = xmid;
-		const build_ret rcl = build_node(c, b, mid, lb, le, depth + 1);
-		const build_ret rcr = build_node(c, mid, e, rb, re, depth + 1);
+		build_ret rcl, rcr;
+		if (nparts > 100000) {
+#pragma omp task shared(rcl) firstprivate(b, mid, depth)
+			rcl = build_node(c, b, mid, lb, le, depth + 1);
+#pragma omp task shared(rcr) firstprivate(mid, e, depth)
+			rcr = build_node(c, mid, e, rb, re, depth + 1);
+#pragma omp taskwait
+		} else {
+			rcl = build_node(c, b, mid, lb, le, depth + 1);
+			rcr = build_node(c, mid, e, rb, re, depth + 1);
+		}
 		if (t->error) {
 			rc.id = -1;
 			return rc;
@@ -927,12 +942,14 @@ static build_ret build_node(build_ctx* c, int64_t b, int64_t e, const double box
 }
 
 int oc_tree_build(oc_tree* t, int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, int32_t* id, int bucket, double theta,
-		double h, int nthreads) {
-	(void) nthreads;
+		double h, int nthreads, int64_t cap_hint) {
 	oc_init();
-	t->nodes = NULL;
+	if (nthreads <= 0) nthreads = 1;
+	/* node budget; the build reports error 2 when it is exceeded and the caller
+	 * retries from the original arrays with a larger budget */
+	t->cap = cap_hint > 0 ? cap_hint : 8 * n / (bucket > 0 ? bucket : 1) + 65536;
+	t->nodes = (oc_node*) malloc(t->cap * sizeof(oc_node));
 	t->nnodes = 0;
-	t->cap = 0;
 	t->error = 0;
 	build_ctx c;
 	c.t = t;
@@ -944,7 +961,10 @@ int oc_tree_build(oc_tree* t, int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, 
 	c.theta = theta;
 	c.h = h;
 	const double bb[3] = { 0, 0, 0 }, be[3] = { 1, 1, 1 };
-	build_ret rc = build_node(&c, 0, n, bb, be, 0);
+	build_ret rc;
+#pragma omp parallel num_threads(nthreads)
+#pragma omp single
+	rc = build_node(&c, 0, n, bb, be, 0);
 	t->root = rc.id;
 	return t->error;
 }
@@ -984,6 +1004,7 @@ typedef struct {
 	int8_t* rung;
 	double *gx, *gy, *gz, *phi;
 	ivec* lists[5]; /* per node, may be NULL */
+	int64_t sample_b, sample_e; /* only sinks in [sample_b, sample_e) are kicked */
 	int error;
 	oc_kick_return ret;
 } kick_ctx;
@@ -1135,6 +1156,7 @@ static void walk(kick_ctx* c, int self_id, const double* Lin, const uint32_t Lpo
 	const oc_tree* t = c->t;
 	const oc_kick_params* p = c->p;
 	const oc_node* self = &t->nodes[self_id];
+	if (self->pend <= c->sample_b || self->pbegin >= c->sample_e) return;
 	const float thetainv = 1.f / p->theta;
 	const float h = p->h;
 	double L[50];
@@ -1340,8 +1362,18 @@ static void walk(kick_ctx* c, int self_id, const double* Lin, const uint32_t Lpo
 	free(ecc.v);
 }
 
+int oc_kick_sample(const oc_tree* t, const oc_kick_params* p, const uint32_t* x, const uint32_t* y, const uint32_t* z, float* vel,
+		int8_t* rung, double* gx, double* gy, double* gz, double* phi, oc_kick_return* ret, oc_lists* lists, int nthreads,
+		int64_t sample_b, int64_t sample_e);
+
 int oc_kick(const oc_tree* t, const oc_kick_params* p, const uint32_t* x, const uint32_t* y, const uint32_t* z, float* vel,
 		int8_t* rung, double* gx, double* gy, double* gz, double* phi, oc_kick_return* ret, oc_lists* lists, int nthreads) {
+	return oc_kick_sample(t, p, x, y, z, vel, rung, gx, gy, gz, phi, ret, lists, nthreads, 0, INT64_MAX);
+}
+
+int oc_kick_sample(const oc_tree* t, const oc_kick_params* p, const uint32_t* x, const uint32_t* y, const uint32_t* z, float* vel,
+		int8_t* rung, double* gx, double* gy, double* gz, double* phi, oc_kick_return* ret, oc_lists* lists, int nthreads,
+		int64_t sample_b, int64_t sample_e) {
 	oc_init();
 	kick_ctx c;
 	memset(&c, 0, sizeof(c));
@@ -1356,6 +1388,8 @@ int oc_kick(const oc_tree* t, const oc_kick_params* p, const uint32_t* x, const 
 	c.gy = gy;
 	c.gz = gz;
 	c.phi = phi;
+	c.sample_b = sample_b;
+	c.sample_e = sample_e;
 	if (lists) {
 		for (int q = 0; q < 5; q++)
 			c.lists[q] = (ivec*) calloc(t->nnodes, sizeof(ivec));

@@ -78,7 +78,7 @@ typedef struct {
 /* Builds the tree over particles [0, n).  Permutes x,y,z and id in place
  * (the partition of src/particles.cpp:979-1018).  Returns 0 on success. */
 int oc_tree_build(oc_tree* t, int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, int32_t* id, int bucket, double theta,
-		double h, int nthreads);
+		double h, int nthreads, int64_t cap_hint);
 void oc_tree_free(oc_tree* t);
 
 /* ---------------- kick ---------------- */
@@ -133,6 +133,11 @@ enum { OC_LIST_EWALD = 0, OC_LIST_CC = 1, OC_LIST_CP = 2, OC_LIST_PC = 3, OC_LIS
 int oc_kick(const oc_tree* t, const oc_kick_params* p, const uint32_t* x, const uint32_t* y, const uint32_t* z,
 		float* vel, int8_t* rung, double* gx, double* gy, double* gz, double* phi, oc_kick_return* ret,
 		oc_lists* lists, int nthreads);
+/* As oc_kick, but only the sinks in the particle range [sample_b, sample_e)
+ * are kicked (a bounded sample of the same walk, for CPU timing). */
+int oc_kick_sample(const oc_tree* t, const oc_kick_params* p, const uint32_t* x, const uint32_t* y, const uint32_t* z,
+		float* vel, int8_t* rung, double* gx, double* gy, double* gz, double* phi, oc_kick_return* ret, oc_lists* lists,
+		int nthreads, int64_t sample_b, int64_t sample_e);
 void oc_lists_free(oc_lists* l);
 
 /* ---------------- drift ---------------- */

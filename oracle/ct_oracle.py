@@ -71,6 +71,8 @@ def _declare(L):
     L.oc_init.restype = None
     L.oc_trless_index.restype = ctypes.c_int
     L.oc_trless_index.argtypes = [ctypes.c_int] * 4
+    L.oc_full.restype = ctypes.c_double
+    L.oc_full.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     for name in ("oc_greens", "oc_ewald_greens"):
         getattr(L, name).argtypes = [_D, _D]
         getattr(L, name).restype = None
@@ -96,12 +98,14 @@ def _declare(L):
     L.oc_fixed32_from_double.argtypes = [ctypes.c_double]
     L.oc_fixed32_from_double.restype = ctypes.c_uint32
     L.oc_tree_build.argtypes = [ctypes.POINTER(Tree), ctypes.c_int64, _U32, _U32, _U32, _I32, ctypes.c_int, ctypes.c_double,
-                                ctypes.c_double, ctypes.c_int]
+                                ctypes.c_double, ctypes.c_int, ctypes.c_int64]
     L.oc_tree_build.restype = ctypes.c_int
     L.oc_tree_free.argtypes = [ctypes.POINTER(Tree)]
     L.oc_kick.argtypes = [ctypes.POINTER(Tree), ctypes.POINTER(KickParams), _U32, _U32, _U32, _F, _I8, _D, _D, _D, _D,
                           ctypes.POINTER(KickReturn), ctypes.POINTER(Lists), ctypes.c_int]
     L.oc_kick.restype = ctypes.c_int
+    L.oc_kick_sample.argtypes = L.oc_kick.argtypes + [ctypes.c_int64, ctypes.c_int64]
+    L.oc_kick_sample.restype = ctypes.c_int
     L.oc_lists_free.argtypes = [ctypes.POINTER(Lists)]
     L.oc_drift.argtypes = [ctypes.c_int64, _U32, _U32, _U32, _F, _I8, ctypes.c_int, ctypes.c_float, ctypes.c_float]
     L.oc_drift.restype = None
@@ -218,14 +222,22 @@ class OracleTree:
     """Tree built by the oracle over copies of the given positions."""
 
     def __init__(self, x, y, z, bucket, theta, h, nthreads=8):
-        self.x = np.array(x, dtype=np.uint32)
-        self.y = np.array(y, dtype=np.uint32)
-        self.z = np.array(z, dtype=np.uint32)
-        self.n = len(self.x)
-        self.id = np.arange(self.n, dtype=np.int32)
-        self.t = Tree()
-        rc = lib().oc_tree_build(ctypes.byref(self.t), self.n, _p(self.x, _U32), _p(self.y, _U32), _p(self.z, _U32),
-                                 _p(self.id, _I32), int(bucket), float(theta), float(h), int(nthreads))
+        x0 = np.array(x, dtype=np.uint32)
+        y0 = np.array(y, dtype=np.uint32)
+        z0 = np.array(z, dtype=np.uint32)
+        self.n = len(x0)
+        cap = 0
+        for _ in range(6):
+            self.x, self.y, self.z = x0.copy(), y0.copy(), z0.copy()
+            self.id = np.arange(self.n, dtype=np.int32)
+            self.t = Tree()
+            rc = lib().oc_tree_build(ctypes.byref(self.t), self.n, _p(self.x, _U32), _p(self.y, _U32), _p(self.z, _U32),
+                                     _p(self.id, _I32), int(bucket), float(theta), float(h), int(nthreads), int(cap))
+            if rc == 2:
+                cap = 2 * self.t.cap
+                lib().oc_tree_free(ctypes.byref(self.t))
+                continue
+            break
         if rc:
             raise RuntimeError("oracle tree build failed: %d" % rc)
         self.nnodes = self.t.nnodes
@@ -248,7 +260,7 @@ class OracleTree:
             pass
 
 
-def kick(tree, params, vel, rung, want_forces=True, want_lists=False, nthreads=8):
+def kick(tree, params, vel, rung, want_forces=True, want_lists=False, nthreads=8, sample=None):
     """Oracle kick over a tree.  vel: (n,3) float32 in tree order (updated in
     place), rung: int8 (updated in place).  Returns (forces dict, kick_return,
     lists)."""
@@ -258,9 +270,10 @@ def kick(tree, params, vel, rung, want_forces=True, want_lists=False, nthreads=8
     g = [np.zeros(n) for _ in range(4)] if want_forces else None
     ret = KickReturn()
     lists = Lists() if want_lists else None
-    rc = lib().oc_kick(ctypes.byref(tree.t), ctypes.byref(params), _p(tree.x, _U32), _p(tree.y, _U32), _p(tree.z, _U32),
-                       _p(vel, _F), _p(rung, _I8), *([_p(a, _D) for a in g] if g else [None] * 4), ctypes.byref(ret),
-                       ctypes.byref(lists) if lists is not None else None, int(nthreads))
+    sb, se = (0, 2 ** 62) if sample is None else sample
+    rc = lib().oc_kick_sample(ctypes.byref(tree.t), ctypes.byref(params), _p(tree.x, _U32), _p(tree.y, _U32), _p(tree.z, _U32),
+                              _p(vel, _F), _p(rung, _I8), *([_p(a, _D) for a in g] if g else [None] * 4), ctypes.byref(ret),
+                              ctypes.byref(lists) if lists is not None else None, int(nthreads), int(sb), int(se))
     if rc:
         raise RuntimeError("oracle kick failed: %d" % rc)
     out_lists = None

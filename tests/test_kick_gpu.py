@@ -1,0 +1,159 @@
+"""GPU kick parity against the oracle (the CPU restatement of src/kick.cu).
+
+Bars:
+  * interaction lists (index work): identical per node and kind, in order;
+  * forces: FP32 GPU vs FP64 oracle, relative |dg|/|g| rms <= 2e-5 and
+    max <= 2e-3; potential error relative to rms(phi) <= 1e-4;
+  * rungs: equal except where the oracle's log2(t0/dt) lies within 1e-4 of
+    an integer (an FP32 rounding can move the ceil);
+  * kick_return: FP64 sums within 1e-4 relative (forces enter in FP32);
+  * drift: positions bit-exact against the oracle's drift of the same input.
+"""
+import numpy as np
+import pytest
+
+from kick_parity import compare_lists, force_errors, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(kind, n, seed):
+    from cosmictiger_amd import workloads
+    if kind == "uniform":
+        return workloads.uniform_box(n, seed)
+    return workloads.nfw_box(n, seed, clustered_fraction=0.7, nhalos=30)
+
+
+@pytest.mark.parametrize("kind,n,theta,bucket", [("uniform", 20000, 0.7, 128), ("clustered", 30000, 0.55, 64),
+                                                  ("clustered", 12000, 0.4, 32)])
+def test_kick_matches_oracle(gpu, oc, kind, n, theta, bucket):
+    x, y, z = _box(kind, n, 11)
+    h = gpu.softening(n)
+    res = run_pair(gpu, oc, x, y, z, None, None, theta, bucket, h, 1.0 / n)
+    assert compare_lists(res) == 0
+    err, perr = force_errors(res)
+    print("force rel err rms %.3e max %.3e; phi %.3e" % (np.sqrt(np.mean(err ** 2)), err.max(), np.sqrt(np.mean(perr ** 2))))
+    assert np.sqrt(np.mean(err ** 2)) <= 2e-5
+    assert err.max() <= 2e-3
+    assert np.sqrt(np.mean(perr ** 2)) <= 1e-4
+    # velocities: first_call skips the ascending half kick, descending adds g dt/2
+    dv = np.abs(res["gpu"]["vel"].astype(np.float64) - res["oracle"]["vel"])
+    vscale = np.max(np.abs(res["oracle"]["vel"]))
+    assert dv.max() <= 2e-3 * vscale
+    # rungs
+    mism = np.nonzero(res["gpu"]["rung"] != res["oracle"]["rung"])[0]
+    assert len(mism) <= max(2, n // 5000)
+    r, o = res["ret"], res["oret"]
+    assert r.max_rung == o.max_rung or len(mism) > 0
+    for f in ("pot", "kin", "dkin", "nmom"):
+        a, b = getattr(r, f), getattr(o, f)
+        assert abs(a - b) <= 1e-4 * max(abs(b), 1e-30), (f, a, b)
+
+
+def test_kick_nonzero_velocities_and_rungs(gpu, oc):
+    """A later kick: ascending and descending half kicks, non-zero rungs."""
+    from cosmictiger_amd import workloads
+    n = 16000
+    x, y, z = workloads.nfw_box(n, 5, clustered_fraction=0.5, nhalos=10)
+    vel = workloads.gaussian_velocities(n, 5)
+    rung = np.random.default_rng(1).integers(0, 3, n).astype(np.int8)
+    h = gpu.softening(n)
+    res = run_pair(gpu, oc, x, y, z, vel, rung, 0.6, 96, h, 1.0 / n, params_kw={"first_call": False, "t0": 0.05, "min_rung": 0})
+    assert compare_lists(res) == 0
+    err, _ = force_errors(res)
+    assert np.sqrt(np.mean(err ** 2)) <= 2e-5
+    dv = np.abs(res["gpu"]["vel"].astype(np.float64) - res["oracle"]["vel"])
+    assert dv.max() <= 1e-5 * max(np.max(np.abs(res["oracle"]["vel"])), 1e-30)
+
+
+def test_kick_without_phi(gpu, oc):
+    n = 8000
+    x, y, z = _box("uniform", n, 3)
+    h = gpu.softening(n)
+    res = run_pair(gpu, oc, x, y, z, None, None, 0.7, 128, h, 1.0 / n, params_kw={"do_phi": False})
+    assert compare_lists(res) == 0
+    err, _ = force_errors(res)
+    assert np.sqrt(np.mean(err ** 2)) <= 2e-5
+
+
+def test_kick_matches_direct_ewald(gpu, oc):
+    """Physics: the GPU kick against the FP64 direct Ewald sum; at theta=0.4
+    on a uniform box the restated algorithm is accurate to a few 1e-4."""
+    n = 20000
+    x, y, z = _box("uniform", n, 21)
+    h = gpu.softening(n)
+    box = gpu.Box(x, y, z)
+    box.tree_build(0.4, 128, h)
+    box.kick(gpu.default_kick_params(0.4, h, 1.0 / n))
+    out = box.read()
+    inv = np.empty(n, dtype=np.int64)
+    inv[out["id"]] = np.arange(n)
+    sinks = np.random.default_rng(2).choice(n, 400, replace=False)
+    ref = oc.direct_ewald(x, y, z, sinks, h, 1.0 / n)
+    g = np.stack([out["gx"][inv][sinks], out["gy"][inv][sinks], out["gz"][inv][sinks]], 1)
+    gr = np.stack([ref["gx"], ref["gy"], ref["gz"]], 1)
+    err = np.linalg.norm(g - gr, axis=1) / np.linalg.norm(gr, axis=1)
+    print("vs direct Ewald: rms %.3e p99 %.3e" % (np.sqrt(np.mean(err ** 2)), np.percentile(err, 99)))
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-3
+    box.close()
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 37])
+def test_tiny_boxes(gpu, oc, n):
+    """Empty and ragged boxes: the kick runs and matches the oracle."""
+    x, y, z = _box("uniform", n, 4)
+    h = gpu.softening(max(n, 1))
+    box = gpu.Box(x, y, z)
+    box.tree_build(0.7, 128, h)
+    r = box.kick(gpu.default_kick_params(0.7, h, 1.0 / max(n, 1)))
+    if n == 0:
+        assert r.kin == 0 and r.pot == 0
+        return
+    res = run_pair(gpu, oc, x, y, z, None, None, 0.7, 128, h, 1.0 / n)
+    assert compare_lists(res) == 0
+    err, perr = force_errors(res)
+    if n > 1:
+        assert err.max() <= 1e-3
+    assert np.max(perr) <= 1e-3
+
+
+def test_drift_bit_exact(gpu, oc):
+    from cosmictiger_amd import workloads
+    n = 50000
+    x, y, z = workloads.nfw_box(n, 9, clustered_fraction=0.5, nhalos=10)
+    vel = workloads.gaussian_velocities(n, 9, sigma=0.05)
+    # particles at the wrap points
+    x[:4] = [0, 1, 2 ** 32 - 1, 2 ** 31]
+    vel[:4, 0] = [-1e-3, -1e-12, 1e-3, 0.0]
+    rung = (np.arange(n) % 3).astype(np.int8)
+    box = gpu.Box(x, y, z, vel, rung)
+    box.drift(1, 0.7, 0.01)
+    out = box.read()
+    inv = np.empty(n, dtype=np.int64)
+    inv[out["id"]] = np.arange(n)
+    ox, oy, oz = oc.drift(x, y, z, vel, rung, 1, 0.7, 0.01)
+    assert np.array_equal(out["x"][inv], ox)
+    assert np.array_equal(out["y"][inv], oy)
+    assert np.array_equal(out["z"][inv], oz)
+    assert np.array_equal(out["x"][inv][rung != 1], x[rung != 1])
+    box.close()
+
+
+def test_device_expansions_match_host(gpu):
+    """FP32 device instantiation of every expansion kernel against the FP64
+    host instantiation."""
+    rng = np.random.default_rng(0)
+    M = np.concatenate([gpu.fmm_host("P2M", rng.uniform(-1e-3, 1e-3, 3)) for _ in range(1)])
+    for op, make in [("greens", lambda: rng.uniform(-0.3, 0.3, 3)), ("ewald_greens", lambda: rng.uniform(-0.5, 0.5, 3)),
+                     ("P2M", lambda: rng.uniform(-1e-3, 1e-3, 3)),
+                     ("M2L", lambda: np.concatenate([M, gpu.fmm_host("greens", rng.uniform(0.02, 0.1, 3))])),
+                     ("M2L2", lambda: np.concatenate([M, gpu.fmm_host("greens", rng.uniform(0.02, 0.1, 3))])),
+                     ("L2L", lambda: np.concatenate([gpu.fmm_host("greens", rng.uniform(0.02, 0.1, 3)), rng.uniform(-2e-3, 2e-3, 3)])),
+                     ("L2P", lambda: np.concatenate([gpu.fmm_host("greens", rng.uniform(0.02, 0.1, 3)), rng.uniform(-2e-3, 2e-3, 3)])),
+                     ("M2M", lambda: np.concatenate([M, rng.uniform(-1e-3, 1e-3, 3)]))]:
+        inp = np.stack([make() for _ in range(64)])
+        dev = gpu.fmm_device(op, inp)
+        host = np.stack([gpu.fmm_host(op, row) for row in inp])
+        scale = np.max(np.abs(host), axis=1, keepdims=True)
+        rel = np.max(np.abs(dev - host) / scale)
+        assert rel <= 3e-5, (op, rel)
