@@ -104,13 +104,14 @@ def _declare(L):
     L.ct_tree_read.argtypes = [_P, _P, ctypes.c_int64]
     L.ct_lists_read.argtypes = [_P, ctypes.c_int, _P, _P, ctypes.c_int64]
     L.ct_kick_stats.argtypes = [_P, _P, ctypes.c_int]
+    L.ct_direct_ewald.argtypes = [_P, ctypes.c_int64, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P]
     L.ct_fmm_host.argtypes = [ctypes.c_int, _P, _P]
     L.ct_fmm_device.argtypes = [ctypes.c_int, ctypes.c_int64, _P, _P]
     L.ct_last_error.argtypes = []
     L.ct_last_error.restype = ctypes.c_char_p
     for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_host", "ct_kick",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
-                 "ct_fmm_device"):
+                 "ct_fmm_device", "ct_direct_ewald"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -266,6 +267,19 @@ class Box:
         idx = np.zeros(max(int(off[-1]), 1), np.int32)
         _check(lib().ct_lists_read(self._h, int(kind), _ptr(off), _ptr(idx), len(idx)))
         return off, idx[:int(off[-1])]
+
+    def direct_ewald(self, sx, sy, sz, hsoft, stream=None):
+        """FP64 direct Ewald sum at the given fixed32 sink positions over every
+        particle of the box, unit masses (gravity_analytic_call_kernel,
+        src/analytic.cu:30).  Returns dict of float64 arrays phi, gx, gy, gz."""
+        sx = np.ascontiguousarray(sx, dtype=np.uint32)
+        sy = np.ascontiguousarray(sy, dtype=np.uint32)
+        sz = np.ascontiguousarray(sz, dtype=np.uint32)
+        m = len(sx)
+        out = {k: np.zeros(m) for k in ("phi", "gx", "gy", "gz")}
+        _check(lib().ct_direct_ewald(self._h, m, _ptr(sx), _ptr(sy), _ptr(sz), float(hsoft), _ptr(out["phi"]), _ptr(out["gx"]),
+                                     _ptr(out["gy"]), _ptr(out["gz"]), stream))
+        return out
 
     def stats(self):
         out = np.zeros(12)

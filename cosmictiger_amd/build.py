@@ -16,6 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 HOST_ARCH = ["-mavx2", "-mfma"]
+HIP_SOURCES = ["ct_kernels.hip", "ct_direct.hip"]
 
 
 def run(cmd):
@@ -36,10 +37,11 @@ def build(regenerate=True):
     tree_o = os.path.join(build_dir, "ct_tree.o")
     run([HIPCC] + COMMON + HOST_ARCH + ["-x", "c++", "-c", os.path.join(CSRC, "ct_tree.cpp"), "-o", tree_o])
     objs.append(tree_o)
-    kern_o = os.path.join(build_dir, "ct_kernels.o")
-    run([HIPCC] + COMMON + DEVICE_FLAGS + ["-Xarch_host", "-mavx2", "-Xarch_host", "-mfma", "-c",
-                                           os.path.join(CSRC, "ct_kernels.hip"), "-o", kern_o])
-    objs.append(kern_o)
+    for src in HIP_SOURCES:
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+        run([HIPCC] + COMMON + DEVICE_FLAGS + ["-Xarch_host", "-mavx2", "-Xarch_host", "-mfma", "-c",
+                                               os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
     run([HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)

@@ -1,0 +1,90 @@
+// Host-side state of a resident box and the error plumbing shared by the
+// HIP sources of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+
+#include "../../include/cosmictiger_amd.h"
+#include "ct_device.hpp"
+#include "ct_tree.hpp"
+
+namespace ct {
+
+// Message of the last failed ABI call on this thread (ct_last_error).
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+	g_err = msg;
+	return code;
+}
+
+#define CT_HIP(call)                                                                                                        \
+	do {                                                                                                                    \
+		hipError_t e_ = (call);                                                                                             \
+		if (e_ != hipSuccess) return ::ct::fail(CT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));            \
+	} while (0)
+
+// Device buffer that only grows.
+template<class T>
+struct DBuf {
+	T* p = nullptr;
+	int64_t n = 0;
+	int grow(int64_t want) {
+		if (want <= n) return CT_OK;
+		if (p) (void) hipFree(p);
+		p = nullptr;
+		n = 0;
+		if (hipMalloc(&p, std::max<int64_t>(want, 1) * sizeof(T)) != hipSuccess) {
+			p = nullptr;
+			return fail(CT_E_OOM, "hipMalloc failed");
+		}
+		n = want;
+		return CT_OK;
+	}
+	void release() {
+		if (p) (void) hipFree(p);
+		p = nullptr;
+		n = 0;
+	}
+};
+
+// ct_init has been called (device selected, Ewald tables uploaded).
+int check_init();
+
+}  // namespace ct
+
+// A resident box: host copy of the particles and tree, device state in HBM.
+struct ct_box {
+	ct::HostParticles hp;
+	ct::HostTree tree;
+	bool have_tree = false;
+	bool host_stale = false;  // device particles are newer than the host copy
+	double theta = 0.7, hsoft = 0.0;
+	int bucket = 128;
+	// device particle state (tree order)
+	ct::DBuf<uint32_t> x, y, z;
+	ct::DBuf<float> vx, vy, vz, gx, gy, gz, phi;
+	ct::DBuf<int8_t> rung;
+	// tree
+	ct::DBuf<ct::DNode> nodes;
+	ct::DBuf<float> multi, L;
+	ct::DBuf<int32_t> leaf_ids;
+	int64_t nnodes = 0, nleaves = 0;
+	// walk
+	ct::DBuf<ct::Act> act[2];
+	ct::DBuf<int32_t> dp[2], ep[2];
+	ct::DBuf<int32_t> out[ct::kNumLists];
+	ct::DBuf<ct::Seg> seg;
+	ct::DBuf<int32_t> head;
+	ct::DBuf<ct::Counters> cnt;
+	ct::DBuf<double> part, red;
+	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[ct::kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
+	// diagnostics
+	hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+	double stats[12] = {0};
+	ct::Counters last_counters;
+	bool have_lists = false;
+};

@@ -33,6 +33,7 @@
 #include "../../include/cosmictiger_amd.h"
 #include "ct_device.hpp"
 #include "ct_expansions.hpp"
+#include "ct_host.hpp"
 #include "ct_tree.hpp"
 
 namespace ct {
@@ -963,17 +964,6 @@ __global__ void k_fmm(int op, int64_t n, const double* __restrict__ in, double* 
 
 thread_local std::string g_err;
 
-int fail(int code, const std::string& msg) {
-	g_err = msg;
-	return code;
-}
-
-#define CT_HIP(call)                                                                                                        \
-	do {                                                                                                                    \
-		hipError_t e_ = (call);                                                                                             \
-		if (e_ != hipSuccess) return fail(CT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));                  \
-	} while (0)
-
 struct EwaldHost {
 	std::vector<float> rimg, fh, fc;
 	int nreal = 0, nfour = 0;
@@ -1057,64 +1047,9 @@ int upload_ewald() {
 	return CT_OK;
 }
 
-template<class T>
-struct DBuf {
-	T* p = nullptr;
-	int64_t n = 0;
-	int grow(int64_t want) {
-		if (want <= n) return CT_OK;
-		if (p) (void) hipFree(p);
-		p = nullptr;
-		n = 0;
-		if (hipMalloc(&p, std::max<int64_t>(want, 1) * sizeof(T)) != hipSuccess) {
-			p = nullptr;
-			return fail(CT_E_OOM, "hipMalloc failed");
-		}
-		n = want;
-		return CT_OK;
-	}
-	void release() {
-		if (p) (void) hipFree(p);
-		p = nullptr;
-		n = 0;
-	}
-};
-
 }  // namespace ct
 
 using namespace ct;
-
-struct ct_box {
-	HostParticles hp;
-	HostTree tree;
-	bool have_tree = false;
-	bool host_stale = false;  // device particles are newer than the host copy
-	double theta = 0.7, hsoft = 0.0;
-	int bucket = 128;
-	// device particle state (tree order)
-	DBuf<uint32_t> x, y, z;
-	DBuf<float> vx, vy, vz, gx, gy, gz, phi;
-	DBuf<int8_t> rung;
-	// tree
-	DBuf<DNode> nodes;
-	DBuf<float> multi, L;
-	DBuf<int32_t> leaf_ids;
-	int64_t nnodes = 0, nleaves = 0;
-	// walk
-	DBuf<Act> act[2];
-	DBuf<int32_t> dp[2], ep[2];
-	DBuf<int32_t> out[kNumLists];
-	DBuf<Seg> seg;
-	DBuf<int32_t> head;
-	DBuf<Counters> cnt;
-	DBuf<double> part, red;
-	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
-	// diagnostics
-	hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-	double stats[12] = {0};
-	Counters last_counters;
-	bool have_lists = false;
-};
 
 namespace {
 
@@ -1172,12 +1107,12 @@ int download_particles(ct_box* b) {
 	return CT_OK;
 }
 
-int check_init() {
+}  // namespace
+
+int ct::check_init() {
 	if (g_dev.device < 0) return fail(CT_E_STATE, "ct_init has not been called");
 	return CT_OK;
 }
-
-}  // namespace
 
 extern "C" {
 

@@ -163,6 +163,15 @@ int ct_fmm_host(int op, const double* in, double* out);
  * (in: n * in_stride doubles, out: n * out_stride doubles, host memory). */
 int ct_fmm_device(int op, int64_t n, const double* in, double* out);
 
+/* FP64 direct-summation periodic Ewald force and potential at nsink sink
+ * positions (fixed32) from every particle of the box, unit mass per particle
+ * (multiply by GM), pairs at zero separation excluded, Ferrers softening h.
+ * Host arrays in and out.  Replaces the analytic force check
+ * gravity_analytic_call_kernel (src/analytic.cu:30-116) used by
+ * analytic_compare (src/analytic.cpp:59-113), in FP64 throughout. */
+int ct_direct_ewald(ct_box_t box, int64_t nsink, const uint32_t* sinkx, const uint32_t* sinky, const uint32_t* sinkz, double h,
+		double* phi, double* gx, double* gy, double* gz, void* stream);
+
 const char* ct_last_error(void);
 
 #ifdef __cplusplus
