@@ -282,10 +282,10 @@ class Box:
         return out
 
     def stats(self):
-        out = np.zeros(14)
-        _check(lib().ct_kick_stats(self._h, _ptr(out), 14))
+        out = np.zeros(15)
+        _check(lib().ct_kick_stats(self._h, _ptr(out), 15))
         keys = ["walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "kick_ms", "pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs",
-                "nodes", "sinks", "pp_ms", "leafkick_ms"]
+                "nodes", "sinks", "pp_ms", "leafkick_ms", "checks"]
         return dict(zip(keys, out.tolist()))
 
 

@@ -86,7 +86,7 @@ struct ct_box {
 	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[ct::kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
 	// diagnostics
 	hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-	double stats[14] = {0};
+	double stats[15] = {0};
 	ct::Counters last_counters;
 	bool have_lists = false;
 };

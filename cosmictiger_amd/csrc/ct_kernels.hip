@@ -217,6 +217,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 	Act* nact_out = A.act[(k + 1) & 1];
 	int32_t* ndp = A.dp[(k + 1) & 1];
 	int32_t* nep = A.ep[(k + 1) & 1];
+	unsigned long long nchecks = 0;
 	for (int b0 = wave * kWalkBatch; b0 < nact; b0 += nwaves * kWalkBatch) {
 		const int nb = min(kWalkBatch, nact - b0);
 		// ---- pass 1: per-entry reservations, entry e's in lane e
@@ -231,6 +232,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
 			const bool sleaf = s.flags & 1;
 			const WalkCounts w = walk_count(A, s, dp, ep, uniform(ae.db), uniform(ae.dn), uniform(ae.eb), uniform(ae.en), lane);
+			nchecks += uniform(ae.dn) + uniform(ae.en);
 			if (w.efail || (sleaf && w.enext)) {
 				if (lane == 0) atomicOr(&C->walk_error, 1);
 				continue;
@@ -438,6 +440,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 			}
 		}
 	}
+	if (lane == 0 && nchecks) atomicAdd(&C->stat[7], nchecks);
 }
 
 struct InitArgs {
@@ -1638,6 +1641,7 @@ int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stre
 	b->stats[9] = double(C.stat[4]);
 	b->stats[10] = double(b->nnodes);
 	b->stats[11] = double(C.stat[6]);
+	b->stats[14] = double(C.stat[7]);
 	CT_HIP(hipEventElapsedTime(&ms, b->ev[3], b->ev[6]));
 	b->stats[12] = ms;
 	CT_HIP(hipEventElapsedTime(&ms, b->ev[6], b->ev[4]));
@@ -1787,7 +1791,7 @@ int ct_lists_read(ct_box_t b, int kind, int64_t* off, int32_t* idx, int64_t idx_
 
 int ct_kick_stats(ct_box_t b, double* out, int n) {
 	if (!b || !out) return fail(CT_E_ARG, "ct_kick_stats: bad arguments");
-	for (int q = 0; q < n && q < 14; q++) out[q] = b->stats[q];
+	for (int q = 0; q < n && q < 15; q++) out[q] = b->stats[q];
 	return CT_OK;
 }
 

@@ -149,7 +149,8 @@ int ct_lists_read(ct_box_t box, int kind, int64_t* off, int32_t* idx, int64_t id
  * [2] L2L sweep, [3] leaf kernel (P2P, M2P, L2P, kick), [4] whole kick.
  * Counters: [5] PP pairs, [6] direct CC, [7] Ewald CC, [8] CP particles,
  * [9] PC sink-multipoles, [10] node visits, [11] sinks; [12] P2P kernels,
- * [13] M2P + L2P + kick kernels (ms; [3] = [12] + [13]). */
+ * [13] M2P + L2P + kick kernels (ms; [3] = [12] + [13]); [14] checks
+ * classified by the walk. */
 int ct_kick_stats(ct_box_t box, double* out, int n);
 
 /* Host evaluation of the expansion kernels in double, for checks of the
