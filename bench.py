@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--cpu-sample-max", type=int, default=262144)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--accuracy-sinks", type=int, default=10000)
+    ap.add_argument("--theta-accurate", type=float, default=0.4,
+                    help="second measurement at an opening angle that meets rms force error <= 1e-3 (0 to skip)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pp_kernel_traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,6 +154,19 @@ def main():
         except Exception:
             traffic = None
     acc = accuracy(box, x, y, z, h, n, args.accuracy_sinks, args.seed + 1) if args.accuracy_sinks > 0 else None
+    at_1e3 = None
+    if args.theta_accurate > 0:
+        # the same box at the opening angle where the method meets rms <= 1e-3
+        box.tree_build(args.theta_accurate, args.bucket, h)
+        pa = ct.default_kick_params(args.theta_accurate, h, 1.0 / n)
+        box.kick(pa)
+        ta = time.perf_counter()
+        na = max(1, args.steps // 2)
+        for _ in range(na):
+            box.kick(pa)
+        ea = time.perf_counter() - ta
+        at_1e3 = {"theta": args.theta_accurate, "value": n * na / ea, "ms_per_step": 1e3 * ea / na, "steps": na,
+                  "accuracy": accuracy(box, x, y, z, h, n, args.accuracy_sinks, args.seed + 1) if args.accuracy_sinks > 0 else None}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(x, y, z, args, h)
@@ -184,6 +199,7 @@ def main():
                      "flops_per_launch": flops, "launch_ms": float(np.mean(pp_ms))},
         "cpu_baseline": cpu,
         "accuracy": acc,
+        "at_rms_1e-3": at_1e3,
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "phases_ms": {k: stats[k] for k in ("walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "pp_ms", "leafkick_ms", "kick_ms")},
         "events": {k: stats[k] for k in ("pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs", "nodes", "sinks", "checks")},

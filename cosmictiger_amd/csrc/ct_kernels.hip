@@ -28,6 +28,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cosmictiger_amd.h"
@@ -775,6 +776,22 @@ __device__ __forceinline__ void pp_pair(uint32_t sx, uint32_t sy, uint32_t sz, u
 	pt -= rinv;
 }
 
+// The same pair without the softening test, for source leaves that cannot
+// come within h of any sink (no pair of the two leaves has r <= h).
+__device__ __forceinline__ void pp_pair_far(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, float& fx, float& fy,
+		float& fz, float& pt) {
+	const float dx = float(int32_t(sx - X));
+	const float dy = float(int32_t(sy - Y));
+	const float dz = float(int32_t(sz - Z));
+	const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+	const float rinv = __builtin_amdgcn_rsqf(r2);
+	const float r3 = rinv * rinv * rinv;
+	fx = __builtin_fmaf(dx, r3, fx);
+	fy = __builtin_fmaf(dy, r3, fy);
+	fz = __builtin_fmaf(dz, r3, fz);
+	pt -= rinv;
+}
+
 // P2P (src/gravity.cu:246-404): one wave per leaf.  Lane (slot, phase) holds
 // sinks slot, slot+S, ... and runs over the sources of its phase; the sources
 // of the leaf's PP list are staged through LDS in tiles of 1024, source i of a
@@ -820,7 +837,10 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const uint32_t* bx = tx + phase * STR;
 	const uint32_t* by = ty + phase * STR;
 	const uint32_t* bz = tz + phase * STR;
-	auto flush = [&]() {
+	// near = true: a tile of sources that may lie within h of a sink (checked
+	// pairs with the softening branch); near = false: the branch-free pair.
+	auto flush = [&](auto near_tag) {
+		constexpr bool near = decltype(near_tag)::value;
 		__syncthreads();
 		const int q = (tn & ~(4 * P - 1)) / P;  // full groups of 4 columns in every row
 		for (int t = 0; t < q; t += 4) {
@@ -829,10 +849,17 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			const uint4 Z4 = *reinterpret_cast<const uint4*>(bz + t);
 #pragma unroll
 			for (int g = 0; g < G; g++) {
-				pp_pair(sx[g], sy[g], sz[g], X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
-				pp_pair(sx[g], sy[g], sz[g], X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
-				pp_pair(sx[g], sy[g], sz[g], X4.z, Y4.z, Z4.z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
-				pp_pair(sx[g], sy[g], sz[g], X4.w, Y4.w, Z4.w, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+				if constexpr (near) {
+					pp_pair(sx[g], sy[g], sz[g], X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair(sx[g], sy[g], sz[g], X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair(sx[g], sy[g], sz[g], X4.z, Y4.z, Z4.z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair(sx[g], sy[g], sz[g], X4.w, Y4.w, Z4.w, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+				} else {
+					pp_pair_far(sx[g], sy[g], sz[g], X4.x, Y4.x, Z4.x, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair_far(sx[g], sy[g], sz[g], X4.y, Y4.y, Z4.y, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair_far(sx[g], sy[g], sz[g], X4.z, Y4.z, Z4.z, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair_far(sx[g], sy[g], sz[g], X4.w, Y4.w, Z4.w, fx[g], fy[g], fz[g], pt[g]);
+				}
 			}
 		}
 		for (int t = q; t < q + 4; t++) {
@@ -840,7 +867,11 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 				const uint32_t X = bx[t], Y = by[t], Z = bz[t];
 #pragma unroll
 				for (int g = 0; g < G; g++) {
-					pp_pair(sx[g], sy[g], sz[g], X, Y, Z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+					if constexpr (near) {
+						pp_pair(sx[g], sy[g], sz[g], X, Y, Z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+					} else {
+						pp_pair_far(sx[g], sy[g], sz[g], X, Y, Z, fx[g], fy[g], fz[g], pt[g]);
+					}
 				}
 			}
 		}
@@ -856,29 +887,57 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 		__syncthreads();
 		tn = 0;
 	};
-	for (int sg = uniform(A.head[int64_t(kPP) * A.nnodes + self]); sg >= 0; sg = uniform(A.seg[sg].next)) {
-		const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
-		for (int j = 0; j < sl; j++) {
-			const int o = uniform(A.out_pp[sb + j]);
-			int pb = uniform(A.nodes[o].pbegin);
-			const int pe = uniform(A.nodes[o].pend);
-			nsrc += pe - pb;
-			while (pb < pe) {
-				const int m = min(kPPTile - tn, pe - pb);
-				for (int t = lane; t < m; t += 64) {
-					const int i = tn + t;
-					const int at = (i % P) * STR + i / P;
-					tx[at] = A.px[pb + t];
-					ty[at] = A.py[pb + t];
-					tz[at] = A.pz[pb + t];
+	// A source leaf is near when its bounding sphere comes within h of the
+	// sink leaf's: dist(centres) - r_s - r_o <= h (plus 1e-6 for the float
+	// rounding of the centres, radii and distance).  Pass 0 stages the far
+	// leaves, pass 1 the near ones.
+	const uint32_t cs0 = uniform(int(A.nodes[self].pos[0])), cs1 = uniform(int(A.nodes[self].pos[1])), cs2 = uniform(int(A.nodes[self].pos[2]));
+	const float rs = A.nodes[self].radius;
+	const int hpp = uniform(A.head[int64_t(kPP) * A.nnodes + self]);
+	for (int pass = 0; pass < 2; pass++) {
+		for (int sg = hpp; sg >= 0; sg = uniform(A.seg[sg].next)) {
+			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			for (int j = 0; j < sl; j++) {
+				const int o = uniform(A.out_pp[sb + j]);
+				const DNode& on = A.nodes[o];
+				const float ddx = fdist(cs0, uint32_t(uniform(int(on.pos[0]))));
+				const float ddy = fdist(cs1, uint32_t(uniform(int(on.pos[1]))));
+				const float ddz = fdist(cs2, uint32_t(uniform(int(on.pos[2]))));
+				const float dd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+				const bool near = dd - (rs + on.radius) <= hf + 1e-6f;
+				if (near != (pass == 1)) continue;
+				int pb = uniform(on.pbegin);
+				const int pe = uniform(on.pend);
+				nsrc += pe - pb;
+				while (pb < pe) {
+					const int m = min(kPPTile - tn, pe - pb);
+					for (int t = lane; t < m; t += 64) {
+						const int i = tn + t;
+						const int at = (i % P) * STR + i / P;
+						tx[at] = A.px[pb + t];
+						ty[at] = A.py[pb + t];
+						tz[at] = A.pz[pb + t];
+					}
+					tn += m;
+					pb += m;
+					if (tn == kPPTile) {
+						if (pass == 0) {
+							flush(std::false_type());
+						} else {
+							flush(std::true_type());
+						}
+					}
 				}
-				tn += m;
-				pb += m;
-				if (tn == kPPTile) flush();
+			}
+		}
+		if (tn) {
+			if (pass == 0) {
+				flush(std::false_type());
+			} else {
+				flush(std::true_type());
 			}
 		}
 	}
-	if (tn) flush();
 #pragma unroll
 	for (int g = 0; g < G; g++) {
 		for (int o = S; o < 64; o <<= 1) {
@@ -896,9 +955,12 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 }
 
 // M2P (src/gravity.cu:157-211), then L2P and the kick (src/kick.cu:53-177):
-// one wave per leaf, a sink per lane.  The multipoles of the PC list are
-// wave-uniform (scalar loads); the sinks' M2P sums are FP64.
+// one wave per leaf.  M2P runs over the (sink, multipole) pairs of the leaf,
+// pair t on lane t mod 64 with sink t mod nsink, so every lane is busy
+// whatever the leaf size; the 4 results of a pair are added to the sink's FP64
+// sums in LDS.  L2P and the kick then run a sink per lane.
 __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
+	__shared__ double acc[4 * kMaxLeafSinks];
 	const int li = blockIdx.x;
 	const int self = uniform(A.leaf_ids[li]);
 	const DNode s = A.nodes[self];
@@ -908,8 +970,32 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 	const ct_kick_params& P_ = A.prm;
 	const int hpc = uniform(A.head[int64_t(kPC) * A.nnodes + self]);
 	unsigned long long npc = 0;
-	for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
-		npc += uniform(A.seg[sg].len);
+	if (hpc >= 0) {
+		for (int k = lane; k < 4 * nsink; k += 64) acc[k] = 0.0;
+		__syncthreads();
+		for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
+			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			npc += sl;
+			const int T = nsink * sl;
+			for (int t = lane; t < T; t += 64) {
+				const int j = t / nsink;
+				const int k = t - j * nsink;
+				const int pi = pbeg + k;
+				const int o = A.out_pc[sb + j];
+				float M[37];
+				uint32_t op[3];
+				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
+				float D[50];
+				ctfmm::greens<float>(D, fdist(A.px[pi], op[0]), fdist(A.py[pi], op[1]), fdist(A.pz[pi], op[2]));
+				float L4[4] = {0.f, 0.f, 0.f, 0.f};
+				ctfmm::M2L2<float>(L4, M, D, P_.do_phi);
+				atomicAdd(&acc[4 * k + 0], -double(1e6f * L4[1]));
+				atomicAdd(&acc[4 * k + 1], -double(1e6f * L4[2]));
+				atomicAdd(&acc[4 * k + 2], -double(1e6f * L4[3]));
+				atomicAdd(&acc[4 * k + 3], double(1e3f * L4[0]));
+			}
+		}
+		__syncthreads();
 	}
 	// ---- L2P + kick, a sink per lane
 	float Ls[50];
@@ -926,24 +1012,13 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 		const int pi = pbeg + k;
 		const float4 pp = A.pp[pi];
 		double gxd = double(pp.x), gyd = double(pp.y), gzd = double(pp.z), phd = double(pp.w);
-		const uint32_t X = A.px[pi], Y = A.py[pi], Z = A.pz[pi];
-		for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
-			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
-			for (int j = 0; j < sl; j++) {
-				const int o = uniform(A.out_pc[sb + j]);
-				float M[37];
-				uint32_t op[3];
-				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
-				float D[50];
-				ctfmm::greens<float>(D, fdist(X, op[0]), fdist(Y, op[1]), fdist(Z, op[2]));
-				float L4[4] = {0.f, 0.f, 0.f, 0.f};
-				ctfmm::M2L2<float>(L4, M, D, P_.do_phi);
-				gxd -= double(1e6f * L4[1]);
-				gyd -= double(1e6f * L4[2]);
-				gzd -= double(1e6f * L4[3]);
-				phd += double(1e3f * L4[0]);
-			}
+		if (hpc >= 0) {
+			gxd += acc[4 * k + 0];
+			gyd += acc[4 * k + 1];
+			gzd += acc[4 * k + 2];
+			phd += acc[4 * k + 3];
 		}
+		const uint32_t X = A.px[pi], Y = A.py[pi], Z = A.pz[pi];
 		float L4[4];
 		ctfmm::L2P<float>(L4, Ls, fdist(X, s.pos[0]), fdist(Y, s.pos[1]), fdist(Z, s.pos[2]), P_.do_phi);
 		float Fphi = float(phd) + 1e3f * L4[0];
@@ -1025,26 +1100,34 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 #define CT_LEAF_CLASSES(X) X(0, 1, 4) X(1, 2, 4) X(2, 3, 4) X(3, 4, 4) X(4, 5, 4) X(5, 6, 4) X(6, 7, 4) X(7, 8, 4) \
 	X(8, 5, 2) X(9, 6, 2) X(10, 7, 2) X(11, 8, 2)
 
-// deterministic FP64 reduction of the per-leaf kick_return partials
-__global__ __launch_bounds__(1024) void k_reduce(const double* __restrict__ part, int n, double* __restrict__ out) {
-	__shared__ double sh[8][1024];
+// Deterministic FP64 reduction of the per-leaf kick_return partials: block b
+// reduces the b-th contiguous slice of the n records (8 doubles each: max_rung
+// by max, the rest by sum) into out[8 b ..]; a second launch with one block
+// reduces those.
+constexpr int kReduceBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part, int n, double* __restrict__ out) {
+	__shared__ double sh[8][256];
 	const int t = threadIdx.x;
+	const int chunk = (n + gridDim.x - 1) / gridDim.x;
+	const int b = blockIdx.x * chunk;
+	const int e = min(n, b + chunk);
 	double acc[8];
 	for (int q = 0; q < 8; q++) acc[q] = 0.0;
-	for (int i = t; i < n; i += 1024) {
+	for (int i = b + t; i < e; i += 256) {
 		acc[0] = fmax(acc[0], part[int64_t(i) * 8]);
 		for (int q = 1; q < 8; q++) acc[q] += part[int64_t(i) * 8 + q];
 	}
 	for (int q = 0; q < 8; q++) sh[q][t] = acc[q];
 	__syncthreads();
-	for (int w = 512; w > 0; w >>= 1) {
+	for (int w = 128; w > 0; w >>= 1) {
 		if (t < w) {
 			sh[0][t] = fmax(sh[0][t], sh[0][t + w]);
 			for (int q = 1; q < 8; q++) sh[q][t] += sh[q][t + w];
 		}
 		__syncthreads();
 	}
-	if (t < 8) out[t] = sh[t][0];
+	if (t < 8) out[int64_t(blockIdx.x) * 8 + t] = sh[t][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1432,7 +1515,7 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 	b->nleaves = int64_t(leaves.size());
 	if ((rc = b->nodes.grow(nn)) || (rc = b->multi.grow(nn * kMultiStride)) || (rc = b->L.grow(nn * kLStride)) ||
 			(rc = b->leaf_ids.grow(std::max<int64_t>(1, b->nleaves))) || (rc = b->part.grow(std::max<int64_t>(1, b->nleaves) * 8)) ||
-			(rc = b->red.grow(8)) || (rc = b->pp.grow(std::max<int64_t>(1, b->hp.n))))
+			(rc = b->red.grow(8 * (kReduceBlocks + 1))) || (rc = b->pp.grow(std::max<int64_t>(1, b->hp.n))))
 		return rc;
 	CT_HIP(hipMemcpy(b->nodes.p, dn.data(), nn * sizeof(DNode), hipMemcpyHostToDevice));
 	CT_HIP(hipMemcpy(b->multi.p, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -1606,7 +1689,8 @@ int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stre
 	la.part = b->part.p;
 	if (b->nleaves) hipLaunchKernelGGL(k_kick, dim3(b->nleaves), dim3(64), 0, st, la);
 	CT_HIP(hipEventRecord(b->ev[4], st));
-	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, b->part.p, int(b->nleaves), b->red.p);
+	hipLaunchKernelGGL(k_reduce, dim3(kReduceBlocks), dim3(256), 0, st, b->part.p, int(b->nleaves), b->red.p + 8);
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, b->red.p + 8, kReduceBlocks, b->red.p);
 	double red[8];
 	CT_HIP(hipMemcpyAsync(red, b->red.p, sizeof(red), hipMemcpyDeviceToHost, st));
 	CT_HIP(hipEventRecord(b->ev[5], st));
