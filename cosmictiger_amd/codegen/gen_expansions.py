@@ -579,9 +579,10 @@ def gen_ewald():
     E("// rimg: 3*nreal image offsets, fh: 3*nfour wave vectors (one of each +-h pair),")
     E("// fc: nfour*50 box-unit coefficients -2 c(h) (2 pi)^l h^n s_l, c(h) = exp(-pi^2 h^2/a^2)/(pi h^2),")
     E("// s_l = (+1, -1, -1, +1)[l %% 4] multiplying (cos, sin, cos, sin) of 2 pi h.X.")
-    E("template<class T>")
+    E("// TC: storage type of the coefficient table (float for FP32 kicks, double for FP64).")
+    E("template<class T, class TC = float>")
     E("CT_HD inline int ewald_greens(T* __restrict__ D, T X0, T X1, T X2, const float* __restrict__ rimg, int nreal,")
-    E("\t\tconst float* __restrict__ fh, const float* __restrict__ fc, int nfour) {")
+    E("\t\tconst float* __restrict__ fh, const TC* __restrict__ fc, int nfour) {")
     E.ind += 1
     for n in outs:
         E("T %s = T(0);" % vname("A", n))
@@ -663,7 +664,7 @@ def gen_ewald():
     E("const T hx = ct_fma(T(fh[3 * i + 0]), X0, ct_fma(T(fh[3 * i + 1]), X1, T(fh[3 * i + 2]) * X2));")
     E("T sn, cn;")
     E("ct_sincos(T(%r) * hx, &sn, &cn);" % (2 * math.pi))
-    E("const float* __restrict__ c = fc + %d * i;" % (P * P + 1))
+    E("const TC* __restrict__ c = fc + %d * i;" % (P * P + 1))
     for n in outs:
         l = deg(n)
         E("%s = ct_fma(T(c[%d]), %s, %s);" % (vname("A", n), trless_index(n, P), "cn" if l % 2 == 0 else "sn", vname("A", n)))

@@ -219,8 +219,11 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 	int32_t* ndp = A.dp[(k + 1) & 1];
 	int32_t* nep = A.ep[(k + 1) & 1];
 	unsigned long long nchecks = 0;
-	for (int b0 = wave * kWalkBatch; b0 < nact; b0 += nwaves * kWalkBatch) {
-		const int nb = min(kWalkBatch, nact - b0);
+	// entries per batch: enough to give every wave work at this level, at most
+	// kWalkBatch (one reservation per list per batch)
+	const int bsz = max(1, min(kWalkBatch, (nact + nwaves - 1) / nwaves));
+	for (int b0 = wave * bsz; b0 < nact; b0 += nwaves * bsz) {
+		const int nb = min(bsz, nact - b0);
 		// ---- pass 1: per-entry reservations, entry e's in lane e
 		int my[R_NUM];
 #pragma unroll
@@ -692,13 +695,14 @@ __global__ __launch_bounds__(64) void k_l2l(const DNode* __restrict__ nodes, flo
 // ---------------------------------------------------------------------------
 
 // Leaf size classes: a leaf of n sinks runs P source phases over S = 64/P
-// sink slots, G = ceil(n/S) sinks per lane.  Leaves of 1..128 sinks use P=4
-// (G=1..8); 129..256 use P=2 (G=5..8).
-constexpr int kNumLeafClasses = 12;
+// sink slots, G = ceil(n/S) sinks per lane.  Leaves of 1..96 sinks use P=4
+// (G=1..6); 97..256 use P=2 (G=4..8), which keeps the register count (and so
+// the occupancy) of the large classes down.
+constexpr int kNumLeafClasses = 11;
 constexpr int kMaxLeafSinks = 256;
 
 __host__ __device__ inline int leaf_class(int n) {
-	return n <= 128 ? (n + 15) / 16 - 1 : 8 + (n + 31) / 32 - 5;
+	return n <= 96 ? (n + 15) / 16 - 1 : 6 + (n + 31) / 32 - 4;
 }
 
 struct PPArgs {
@@ -742,7 +746,7 @@ struct LeafArgs {
 	Counters* cnt;
 };
 
-constexpr int kPPTile = 1024;
+constexpr int kPPTile = 640;  // sources per LDS tile (multiple of 16; ~8 KB per block)
 
 // One softened pair in integer units (Delta = sink - source in units of 2^-32),
 // g -= Delta/r^3, phi -= 1/r; Ferrers n=4 inside h (cosmictiger/gravity.hpp:76-96).
@@ -794,7 +798,7 @@ __device__ __forceinline__ void pp_pair_far(uint32_t sx, uint32_t sy, uint32_t s
 
 // P2P (src/gravity.cu:246-404): one wave per leaf.  Lane (slot, phase) holds
 // sinks slot, slot+S, ... and runs over the sources of its phase; the sources
-// of the leaf's PP list are staged through LDS in tiles of 1024, source i of a
+// of the leaf's PP list are staged through LDS in tiles of kPPTile, source i of a
 // tile at row i%P, column i/P (rows padded by 16 words so the P rows sit in
 // different banks).  Each tile is summed in FP32, then added to FP64 sink
 // accumulators (compensated at tile granularity); the phases are summed at the
@@ -839,41 +843,45 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const uint32_t* bz = tz + phase * STR;
 	// near = true: a tile of sources that may lie within h of a sink (checked
 	// pairs with the softening branch); near = false: the branch-free pair.
-	auto flush = [&](auto near_tag) {
+	auto flush = [&](auto near_tag) __attribute__((always_inline)) {
 		constexpr bool near = decltype(near_tag)::value;
 		__syncthreads();
 		const int q = (tn & ~(4 * P - 1)) / P;  // full groups of 4 columns in every row
-		for (int t = 0; t < q; t += 4) {
-			const uint4 X4 = *reinterpret_cast<const uint4*>(bx + t);
-			const uint4 Y4 = *reinterpret_cast<const uint4*>(by + t);
-			const uint4 Z4 = *reinterpret_cast<const uint4*>(bz + t);
+		// sink-major: the tile is read once per sink so that the temporaries do
+		// not grow with G (LDS reads are ~4% of the VALU cycles)
 #pragma unroll
-			for (int g = 0; g < G; g++) {
+		for (int g = 0; g < G; g++) {
+			float ax = fx[g], ay = fy[g], az = fz[g], ap = pt[g];
+			const uint32_t X0 = sx[g], Y0 = sy[g], Z0 = sz[g];
+			for (int t = 0; t < q; t += 4) {
+				const uint4 X4 = *reinterpret_cast<const uint4*>(bx + t);
+				const uint4 Y4 = *reinterpret_cast<const uint4*>(by + t);
+				const uint4 Z4 = *reinterpret_cast<const uint4*>(bz + t);
 				if constexpr (near) {
-					pp_pair(sx[g], sy[g], sz[g], X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
-					pp_pair(sx[g], sy[g], sz[g], X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
-					pp_pair(sx[g], sy[g], sz[g], X4.z, Y4.z, Z4.z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
-					pp_pair(sx[g], sy[g], sz[g], X4.w, Y4.w, Z4.w, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair(X0, Y0, Z0, X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, X4.z, Y4.z, Z4.z, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, X4.w, Y4.w, Z4.w, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
 				} else {
-					pp_pair_far(sx[g], sy[g], sz[g], X4.x, Y4.x, Z4.x, fx[g], fy[g], fz[g], pt[g]);
-					pp_pair_far(sx[g], sy[g], sz[g], X4.y, Y4.y, Z4.y, fx[g], fy[g], fz[g], pt[g]);
-					pp_pair_far(sx[g], sy[g], sz[g], X4.z, Y4.z, Z4.z, fx[g], fy[g], fz[g], pt[g]);
-					pp_pair_far(sx[g], sy[g], sz[g], X4.w, Y4.w, Z4.w, fx[g], fy[g], fz[g], pt[g]);
+					pp_pair_far(X0, Y0, Z0, X4.x, Y4.x, Z4.x, ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, X4.y, Y4.y, Z4.y, ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, X4.z, Y4.z, Z4.z, ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, X4.w, Y4.w, Z4.w, ax, ay, az, ap);
 				}
 			}
-		}
-		for (int t = q; t < q + 4; t++) {
-			if (t * P + phase < tn) {
-				const uint32_t X = bx[t], Y = by[t], Z = bz[t];
-#pragma unroll
-				for (int g = 0; g < G; g++) {
+			for (int t = q; t < q + 4; t++) {
+				if (t * P + phase < tn) {
 					if constexpr (near) {
-						pp_pair(sx[g], sy[g], sz[g], X, Y, Z, h2, ihinv, ih2inv, ih3inv, fx[g], fy[g], fz[g], pt[g]);
+						pp_pair(X0, Y0, Z0, bx[t], by[t], bz[t], h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
 					} else {
-						pp_pair_far(sx[g], sy[g], sz[g], X, Y, Z, fx[g], fy[g], fz[g], pt[g]);
+						pp_pair_far(X0, Y0, Z0, bx[t], by[t], bz[t], ax, ay, az, ap);
 					}
 				}
 			}
+			fx[g] = ax;
+			fy[g] = ay;
+			fz[g] = az;
+			pt[g] = ap;
 		}
 		// integer units -> box units: force 2^64, potential 2^32
 #pragma unroll
@@ -894,7 +902,32 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const uint32_t cs0 = uniform(int(A.nodes[self].pos[0])), cs1 = uniform(int(A.nodes[self].pos[1])), cs2 = uniform(int(A.nodes[self].pos[2]));
 	const float rs = A.nodes[self].radius;
 	const int hpp = uniform(A.head[int64_t(kPP) * A.nnodes + self]);
+	// stage particles [pb, pe) into the tile, flushing whenever it fills
+	auto stage = [&](int pb, const int pe, bool nearpass) __attribute__((always_inline)) {
+		while (pb < pe) {
+			const int m = min(kPPTile - tn, pe - pb);
+#pragma unroll 4
+			for (int t = lane; t < m; t += 64) {
+				const int i = tn + t;
+				const int at = (i % P) * STR + i / P;
+				tx[at] = A.px[pb + t];
+				ty[at] = A.py[pb + t];
+				tz[at] = A.pz[pb + t];
+			}
+			tn += m;
+			pb += m;
+			if (tn == kPPTile) {
+				if (nearpass) {
+					flush(std::true_type());
+				} else {
+					flush(std::false_type());
+				}
+			}
+		}
+	};
 	for (int pass = 0; pass < 2; pass++) {
+		// source leaves adjacent in memory are merged into one range
+		int rb = 0, re = 0;
 		for (int sg = hpp; sg >= 0; sg = uniform(A.seg[sg].next)) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
 			for (int j = 0; j < sl; j++) {
@@ -906,30 +939,19 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 				const float dd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
 				const bool near = dd - (rs + on.radius) <= hf + 1e-6f;
 				if (near != (pass == 1)) continue;
-				int pb = uniform(on.pbegin);
+				const int pb = uniform(on.pbegin);
 				const int pe = uniform(on.pend);
 				nsrc += pe - pb;
-				while (pb < pe) {
-					const int m = min(kPPTile - tn, pe - pb);
-					for (int t = lane; t < m; t += 64) {
-						const int i = tn + t;
-						const int at = (i % P) * STR + i / P;
-						tx[at] = A.px[pb + t];
-						ty[at] = A.py[pb + t];
-						tz[at] = A.pz[pb + t];
-					}
-					tn += m;
-					pb += m;
-					if (tn == kPPTile) {
-						if (pass == 0) {
-							flush(std::false_type());
-						} else {
-							flush(std::true_type());
-						}
-					}
+				if (pb == re) {
+					re = pe;
+				} else {
+					stage(rb, re, pass == 1);
+					rb = pb;
+					re = pe;
 				}
 			}
 		}
+		stage(rb, re, pass == 1);
 		if (tn) {
 			if (pass == 0) {
 				flush(std::false_type());
@@ -1097,8 +1119,8 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 	}
 }
 
-#define CT_LEAF_CLASSES(X) X(0, 1, 4) X(1, 2, 4) X(2, 3, 4) X(3, 4, 4) X(4, 5, 4) X(5, 6, 4) X(6, 7, 4) X(7, 8, 4) \
-	X(8, 5, 2) X(9, 6, 2) X(10, 7, 2) X(11, 8, 2)
+#define CT_LEAF_CLASSES(X) X(0, 1, 4) X(1, 2, 4) X(2, 3, 4) X(3, 4, 4) X(4, 5, 4) X(5, 6, 4) \
+	X(6, 4, 2) X(7, 5, 2) X(8, 6, 2) X(9, 7, 2) X(10, 8, 2)
 
 // Deterministic FP64 reduction of the per-leaf kick_return partials: block b
 // reduces the b-th contiguous slice of the n records (8 doubles each: max_rung
@@ -1575,7 +1597,7 @@ static int run_walk(ct_box* b, hipStream_t st, const ct_kick_params* p) {
 		A.cnt = b->cnt.p;
 		A.theta_inv = 1.0f / p->theta;
 		A.h = p->h;
-		const int grid = 2048;
+		const int grid = 1024;  // 4096 waves, all resident at the walk's register use
 		for (int k = 0; k < iters; k++) {
 			hipLaunchKernelGGL(k_walk, dim3(grid), dim3(256), 0, st, A, k);
 		}
