@@ -45,14 +45,20 @@ def make_box(args):
     return x, y, z
 
 
-def accuracy(box, x, y, z, h, n, nsinks, seed):
+def direct_reference(box, x, y, z, h, n, nsinks, seed):
+    """FP64 direct Ewald forces (device, ct_direct_ewald) on a seeded sample of
+    sinks: (sink indices, reference) for accuracy()."""
+    sinks = np.random.default_rng(seed).choice(n, min(nsinks, n), replace=False)
+    return sinks, box.direct_ewald(x[sinks], y[sinks], z[sinks], h)
+
+
+def accuracy(box, x, y, z, h, n, nsinks, seed, ref=None):
     """Relative force error of the last kick against the FP64 direct Ewald sum
-    (device, ct_direct_ewald) on a seeded sample of sinks."""
+    on a seeded sample of sinks."""
     out = box.read()
     inv = np.empty(n, dtype=np.int64)
     inv[out["id"]] = np.arange(n)
-    sinks = np.random.default_rng(seed).choice(n, min(nsinks, n), replace=False)
-    ref = box.direct_ewald(x[sinks], y[sinks], z[sinks], h)
+    sinks, ref = ref if ref is not None else direct_reference(box, x, y, z, h, n, nsinks, seed)
     gm = 1.0 / n
     g = np.stack([out["gx"][inv][sinks], out["gy"][inv][sinks], out["gz"][inv][sinks]], 1).astype(np.float64)
     gr = gm * np.stack([ref["gx"], ref["gy"], ref["gz"]], 1)
@@ -62,6 +68,31 @@ def accuracy(box, x, y, z, h, n, nsinks, seed):
     return {"sinks": int(len(sinks)), "rms": float(np.sqrt(np.mean(err ** 2))), "p50": float(np.percentile(err, 50)),
             "p99": float(np.percentile(err, 99)), "max": float(err.max()), "mag_p50": float(np.percentile(merr, 50)),
             "mag_p99": float(np.percentile(merr, 99)), "oracle": "FP64 direct Ewald (alpha=9, |h|^2<=220), device"}
+
+
+def precision_sweep(ct, box, x, y, z, h, n, args):
+    """The same box kicked in the three arithmetic modes at three opening
+    angles; one JSON line with the throughput and force errors of each."""
+    ref = direct_reference(box, x, y, z, h, n, args.accuracy_sinks, args.seed + 1)
+    rows = []
+    modes = [("fp32_compensated", ct.PREC_F32C), ("fp64", ct.PREC_F64), ("fp32_uncompensated", ct.PREC_F32U)]
+    for theta in (0.4, 0.55, 0.7):
+        box.tree_build(theta, args.bucket, h)
+        p = ct.default_kick_params(theta, h, 1.0 / n)
+        for name, mode in modes:
+            box.kick(p, precision=mode)
+            t1 = time.perf_counter()
+            box.kick(p, precision=mode)
+            dt = time.perf_counter() - t1
+            acc = accuracy(box, x, y, z, h, n, args.accuracy_sinks, args.seed + 1, ref=ref)
+            rows.append({"theta": theta, "mode": name, "value": n / dt, "ms_per_step": 1e3 * dt, "rms": acc["rms"],
+                         "p50": acc["p50"], "p99": acc["p99"], "max": acc["max"]})
+            print("sweep: theta %.2f %-18s %8.2f ms  rms %.3e  p99 %.3e" % (theta, name, 1e3 * dt, acc["rms"], acc["p99"]),
+                  file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "precision sweep: rms / p99 relative force error vs FP64 direct Ewald", "unit": "particle-kicks/s",
+                      "n_gpus": 1, "data": "synthetic", "config": {"workload": "clustered box %d particles" % n, "n": n,
+                                                                    "bucket": args.bucket, "sinks": int(len(ref[0]))},
+                      "sweep": rows}), flush=True)
 
 
 def cpu_baseline(x, y, z, args, h):
@@ -107,6 +138,9 @@ def main():
     ap.add_argument("--accuracy-sinks", type=int, default=10000)
     ap.add_argument("--theta-accurate", type=float, default=0.4,
                     help="second measurement at an opening angle that meets rms force error <= 1e-3 (0 to skip)")
+    ap.add_argument("--precision-sweep", action="store_true",
+                    help="instead of the throughput line: FP32 compensated / FP64 / FP32 uncompensated kicks at theta "
+                         "0.4, 0.55, 0.7 with their rms and p99 force errors")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pp_kernel_traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -126,6 +160,10 @@ def main():
     tree_s = time.perf_counter() - tb0
     params = ct.default_kick_params(args.theta, h, 1.0 / n)
     print("bench: %d particles, tree %.2fs, setup %.2fs" % (n, tree_s, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    if args.precision_sweep:
+        precision_sweep(ct, box, x, y, z, h, n, args)
+        box.close()
+        return
     for _ in range(args.warmup):
         box.kick(params)
     pp_ms = []

@@ -31,6 +31,11 @@ CT_E_STATE = -5
 CT_E_TREE = -6
 CT_E_WALK = -7
 
+# arithmetic modes of ct_kick_prec
+PREC_F32C = 0  # FP32, FP64 sink accumulation per source tile (the default kick)
+PREC_F32U = 1  # FP32, FP32 running sums
+PREC_F64 = 2   # FP64 everywhere but the walk's acceptance tests
+
 SELF_PHI = 693.0 / 256.0  # cosmictiger/gravity.hpp:98-102
 EXPANSION_SIZE = 50
 MULTIPOLE_SIZE = 37
@@ -98,6 +103,7 @@ def _declare(L):
     L.ct_tree_build_host.argtypes = [ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_double, _P,
                                      ctypes.c_int64, _I64P]
     L.ct_kick.argtypes = [_P, ctypes.POINTER(KickParams), ctypes.POINTER(KickReturn), _P]
+    L.ct_kick_prec.argtypes = [_P, ctypes.POINTER(KickParams), ctypes.c_int, ctypes.POINTER(KickReturn), _P]
     L.ct_drift.argtypes = [_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_double, _P]
     L.ct_box_size.argtypes = [_P, _I64P, _I64P]
     L.ct_box_read.argtypes = [_P] + [_P] * 10
@@ -110,6 +116,7 @@ def _declare(L):
     L.ct_last_error.argtypes = []
     L.ct_last_error.restype = ctypes.c_char_p
     for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_host", "ct_kick",
+                 "ct_kick_prec",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
                  "ct_fmm_device", "ct_direct_ewald"):
         getattr(L, name).restype = ctypes.c_int
@@ -231,9 +238,14 @@ class Box:
     def tree_build(self, theta, bucket, hsoft):
         _check(lib().ct_tree_build(self._h, float(theta), int(bucket), float(hsoft)))
 
-    def kick(self, params, stream=None):
+    def kick(self, params, stream=None, precision=None):
+        """One kick of the box; precision: None (the default FP32 kick) or one
+        of PREC_F32C, PREC_F32U, PREC_F64."""
         ret = KickReturn()
-        _check(lib().ct_kick(self._h, ctypes.byref(params), ctypes.byref(ret), stream))
+        if precision is None:
+            _check(lib().ct_kick(self._h, ctypes.byref(params), ctypes.byref(ret), stream))
+        else:
+            _check(lib().ct_kick_prec(self._h, ctypes.byref(params), int(precision), ctypes.byref(ret), stream))
         return ret
 
     def drift(self, rung, a, dt, drag=0.0, stream=None):

@@ -73,7 +73,9 @@ struct ct_box {
 	ct::DBuf<float> multi, L;
 	ct::DBuf<int32_t> leaf_ids;  // non-empty leaves grouped by size class
 	int64_t leaf_class_off[13] = {0};
-	ct::DBuf<float4> pp;         // P2P result per particle
+	ct::DBuf<double4> pp;        // P2P result per particle (float4 or double4 records)
+	ct::DBuf<double> multi64, L64;  // FP64 multipoles and expansions (FP64 kicks only)
+	bool multi64_ready = false;
 	int64_t nnodes = 0, nleaves = 0;
 	// walk
 	ct::DBuf<ct::Act> act[2];

@@ -51,6 +51,7 @@ constexpr int kNFourMax = 64;
 __constant__ float c_rimg[3 * kNRealMax];
 __constant__ float c_fh[3 * kNFourMax];
 __constant__ float c_fc[kNFourMax * 50];
+__constant__ double c_fc64[kNFourMax * 50];
 __constant__ int c_nreal;
 __constant__ int c_nfour;
 __constant__ float c_rung_dt[16];
@@ -494,10 +495,45 @@ __global__ void k_walk_seed_counts(Counters* C) {
 // cell interactions: thread per node
 // ---------------------------------------------------------------------------
 
+// Arithmetic modes of the interaction kernels (the walk is the same in all):
+//   kF32C  FP32 arithmetic, FP64 sink sums per tile (compensated, the default);
+//   kF32U  FP32 arithmetic and FP32 running sums (uncompensated);
+//   kF64   FP64 arithmetic throughout (multipoles, expansions, P2P, Ewald).
+enum { kF32C = 0, kF32U = 1, kF64 = 2 };
+template<int PR> struct Prec {
+	using T = float;   // arithmetic
+	using A = double;  // sink accumulators
+};
+template<> struct Prec<kF32U> {
+	using T = float;
+	using A = float;
+};
+template<> struct Prec<kF64> {
+	using T = double;
+	using A = double;
+};
+
+// minimum-image difference a - b in box units (cosmictiger/fixed.hpp:303-351)
+template<class T> __device__ __forceinline__ T tdist(uint32_t a, uint32_t b);
+template<> __device__ __forceinline__ float tdist<float>(uint32_t a, uint32_t b) {
+	return fdist(a, b);
+}
+template<> __device__ __forceinline__ double tdist<double>(uint32_t a, uint32_t b) {
+	return double(int32_t(a - b)) * (1.0 / 4294967296.0);
+}
+
+template<class T> __device__ __forceinline__ const T* ewald_fc();
+template<> __device__ __forceinline__ const float* ewald_fc<float>() {
+	return c_fc;
+}
+template<> __device__ __forceinline__ const double* ewald_fc<double>() {
+	return c_fc64;
+}
+
 struct CellArgs {
 	const DNode* nodes;
-	const float* multi;
-	float* L;
+	const void* multi;  // float or double records of kMultiStride
+	void* L;            // float or double, kLStride per node
 	const int32_t* head;
 	const Seg* seg;
 	const int32_t* out_e;
@@ -531,6 +567,25 @@ __device__ __forceinline__ void load_multi(const float* __restrict__ mp, float* 
 	pos[2] = __float_as_uint(tmp[39]);
 }
 
+// FP64 record: 37 moments, then the fixed32 centre as three exact doubles.
+__device__ __forceinline__ void load_multi(const double* __restrict__ mp, double* M, uint32_t* pos) {
+	const double2* m2 = reinterpret_cast<const double2*>(mp);
+	double tmp[40];
+#pragma unroll
+	for (int q = 0; q < 20; q++) {
+		const double2 v = m2[q];
+		tmp[2 * q + 0] = v.x;
+		tmp[2 * q + 1] = v.y;
+	}
+#pragma unroll
+	for (int q = 0; q < 37; q++) {
+		M[q] = tmp[q];
+	}
+	pos[0] = uint32_t(tmp[37]);
+	pos[1] = uint32_t(tmp[38]);
+	pos[2] = uint32_t(tmp[39]);
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 	for (int o = 32; o > 0; o >>= 1) {
 		v += __shfl_xor(v, o);
@@ -539,7 +594,8 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // Sum of v over the 64 lanes of the wave, in every lane.
-__device__ __forceinline__ float wave_sum_f(float v) {
+template<class T>
+__device__ __forceinline__ T wave_sum_f(T v) {
 	for (int o = 32; o > 0; o >>= 1) {
 		v += __shfl_xor(v, o);
 	}
@@ -548,11 +604,12 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 
 // Reduces the per-lane partial expansions over the wave and adds them to the
 // node's L (lane q < 50 writes component q).
-__device__ __forceinline__ void wave_add_expansion(float* __restrict__ Lo, float (&L)[50], int lane, bool overwrite) {
-	float mine = 0.0f;
+template<class T>
+__device__ __forceinline__ void wave_add_expansion(T* __restrict__ Lo, T (&L)[50], int lane, bool overwrite) {
+	T mine = T(0);
 #pragma unroll
 	for (int q = 0; q < 50; q++) {
-		const float t = wave_sum_f(L[q]);
+		const T t = wave_sum_f(L[q]);
 		mine = lane == q ? t : mine;
 	}
 	if (lane < 50) {
@@ -565,30 +622,33 @@ constexpr int kCellTile = 512;
 // Ewald CC (src/gravity.cu:213-244): one wave per sink node, a source cell per
 // lane, then a wave reduction of the 50-float expansion.  Writes L of every
 // node (zero when the node has no Ewald list), so it runs first.
+template<class T>
 __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 	const int lane = threadIdx.x;
+	const T* multi = static_cast<const T*>(A.multi);
 	unsigned long long nev = 0;
 	for (int i = blockIdx.x; i < A.nnodes; i += gridDim.x) {
-		float* Lo = A.L + int64_t(i) * kLStride;
+		T* Lo = static_cast<T*>(A.L) + int64_t(i) * kLStride;
 		const int h0 = uniform(A.head[int64_t(kEwald) * A.nnodes + i]);
 		if (h0 < 0) {
-			if (lane < 50) Lo[lane] = 0.0f;
+			if (lane < 50) Lo[lane] = T(0);
 			continue;
 		}
 		const uint32_t sp0 = uniform(int(A.nodes[i].pos[0])), sp1 = uniform(int(A.nodes[i].pos[1])), sp2 = uniform(int(A.nodes[i].pos[2]));
-		float L[50];
+		T L[50];
 #pragma unroll
-		for (int q = 0; q < 50; q++) L[q] = 0.0f;
+		for (int q = 0; q < 50; q++) L[q] = T(0);
 		for (int sg = h0; sg >= 0; sg = uniform(A.seg[sg].next)) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
 			for (int j = lane; j < sl; j += 64) {
 				const int o = A.out_e[sb + j];
-				float M[37];
+				T M[37];
 				uint32_t op[3];
-				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
-				float D[50];
-				ctfmm::ewald_greens<float>(D, fdist(sp0, op[0]), fdist(sp1, op[1]), fdist(sp2, op[2]), c_rimg, c_nreal, c_fh, c_fc, c_nfour);
-				ctfmm::M2L<float>(L, M, D, A.do_phi);
+				load_multi(multi + int64_t(o) * kMultiStride, M, op);
+				T D[50];
+				ctfmm::ewald_greens<T, T>(D, tdist<T>(sp0, op[0]), tdist<T>(sp1, op[1]), tdist<T>(sp2, op[2]), c_rimg, c_nreal, c_fh, ewald_fc<T>(),
+						c_nfour);
+				ctfmm::M2L<T>(L, M, D, A.do_phi);
 				nev++;
 			}
 		}
@@ -601,28 +661,30 @@ __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 // Direct CC = M2L (src/gravity.cu:44-73), a source cell per lane, and CP = P2L
 // (src/gravity.cu:75-155), the particles of the CP leaves staged through LDS
 // and a source particle per lane; one wave per sink node.
+template<class T>
 __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 	__shared__ uint32_t tx[kCellTile], ty[kCellTile], tz[kCellTile];
 	const int lane = threadIdx.x;
+	const T* multi = static_cast<const T*>(A.multi);
 	unsigned long long ncc = 0, ncp = 0;
 	for (int i = blockIdx.x; i < A.nnodes; i += gridDim.x) {
 		const int hcc = uniform(A.head[int64_t(kCC) * A.nnodes + i]);
 		const int hcp = uniform(A.head[int64_t(kCP) * A.nnodes + i]);
 		if (hcc < 0 && hcp < 0) continue;
 		const uint32_t sp0 = uniform(int(A.nodes[i].pos[0])), sp1 = uniform(int(A.nodes[i].pos[1])), sp2 = uniform(int(A.nodes[i].pos[2]));
-		float L[50];
+		T L[50];
 #pragma unroll
-		for (int q = 0; q < 50; q++) L[q] = 0.0f;
+		for (int q = 0; q < 50; q++) L[q] = T(0);
 		for (int sg = hcc; sg >= 0; sg = uniform(A.seg[sg].next)) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
 			for (int j = lane; j < sl; j += 64) {
 				const int o = A.out_cc[sb + j];
-				float M[37];
+				T M[37];
 				uint32_t op[3];
-				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
-				float D[50];
-				ctfmm::greens<float>(D, fdist(sp0, op[0]), fdist(sp1, op[1]), fdist(sp2, op[2]));
-				ctfmm::M2L<float>(L, M, D, A.do_phi);
+				load_multi(multi + int64_t(o) * kMultiStride, M, op);
+				T D[50];
+				ctfmm::greens<T>(D, tdist<T>(sp0, op[0]), tdist<T>(sp1, op[1]), tdist<T>(sp2, op[2]));
+				ctfmm::M2L<T>(L, M, D, A.do_phi);
 				ncc++;
 			}
 		}
@@ -630,8 +692,8 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 		auto flush = [&]() {
 			__syncthreads();
 			for (int t = lane; t < tn; t += 64) {
-				float D[50];
-				ctfmm::greens<float>(D, fdist(sp0, tx[t]), fdist(sp1, ty[t]), fdist(sp2, tz[t]));
+				T D[50];
+				ctfmm::greens<T>(D, tdist<T>(sp0, tx[t]), tdist<T>(sp1, ty[t]), tdist<T>(sp2, tz[t]));
 #pragma unroll
 				for (int q = 0; q < 50; q++) L[q] += D[q];
 			}
@@ -659,7 +721,7 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 			}
 		}
 		if (tn) flush();
-		wave_add_expansion(A.L + int64_t(i) * kLStride, L, lane, false);
+		wave_add_expansion(static_cast<T*>(A.L) + int64_t(i) * kLStride, L, lane, false);
 	}
 	ncc = wave_sum_u64(ncc);
 	ncp = wave_sum_u64(ncp);
@@ -673,19 +735,20 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 // L2L sweep, one launch per level
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(64) void k_l2l(const DNode* __restrict__ nodes, float* __restrict__ L, int begin, int end, bool do_phi) {
+template<class T>
+__global__ __launch_bounds__(64) void k_l2l(const DNode* __restrict__ nodes, T* __restrict__ L, int begin, int end, bool do_phi) {
 	const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= end) return;
 	const DNode s = nodes[i];
 	if (s.parent < 0) return;
 	const DNode p = nodes[s.parent];
-	const float* La = L + int64_t(s.parent) * kLStride;
-	float Lp[50];
+	const T* La = L + int64_t(s.parent) * kLStride;
+	T Lp[50];
 #pragma unroll
 	for (int q = 0; q < 50; q++) Lp[q] = La[q];
-	float Lb[50];
-	ctfmm::L2L<float>(Lb, Lp, fdist(s.pos[0], p.pos[0]), fdist(s.pos[1], p.pos[1]), fdist(s.pos[2], p.pos[2]), do_phi);
-	float* Lo = L + int64_t(i) * kLStride;
+	T Lb[50];
+	ctfmm::L2L<T>(Lb, Lp, tdist<T>(s.pos[0], p.pos[0]), tdist<T>(s.pos[1], p.pos[1]), tdist<T>(s.pos[2], p.pos[2]), do_phi);
+	T* Lo = L + int64_t(i) * kLStride;
 #pragma unroll
 	for (int q = 0; q < 50; q++) Lo[q] += Lb[q];
 }
@@ -715,7 +778,7 @@ struct PPArgs {
 	const uint32_t* px;
 	const uint32_t* py;
 	const uint32_t* pz;
-	float4* pp;  // per particle: P2P force (gx, gy, gz) and potential, box units
+	void* pp;  // per particle: P2P force (gx, gy, gz) and potential, box units (float4, or double4 in FP64)
 	float h;
 	Counters* cnt;
 };
@@ -723,8 +786,8 @@ struct PPArgs {
 struct LeafArgs {
 	const DNode* nodes;
 	const int32_t* leaf_ids;  // non-empty leaves
-	const float* multi;
-	const float* L;
+	const void* multi;
+	const void* L;
 	const int32_t* head;
 	const Seg* seg;
 	const int32_t* out_pc;
@@ -732,7 +795,7 @@ struct LeafArgs {
 	const uint32_t* px;
 	const uint32_t* py;
 	const uint32_t* pz;
-	const float4* pp;
+	const void* pp;
 	float* vx;
 	float* vy;
 	float* vz;
@@ -750,50 +813,63 @@ constexpr int kPPTile = 640;  // sources per LDS tile (multiple of 16; ~8 KB per
 
 // One softened pair in integer units (Delta = sink - source in units of 2^-32),
 // g -= Delta/r^3, phi -= 1/r; Ferrers n=4 inside h (cosmictiger/gravity.hpp:76-96).
-__device__ __forceinline__ void pp_pair(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, float h2, float hinv,
-		float h2inv, float h3inv, float& fx, float& fy, float& fz, float& pt) {
-	const float dx = float(int32_t(sx - X));
-	const float dy = float(int32_t(sy - Y));
-	const float dz = float(int32_t(sz - Z));
-	const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-	float rinv = __builtin_amdgcn_rsqf(r2);
-	float r3 = rinv * rinv * rinv;
+template<class T>
+__device__ __forceinline__ void pp_pair(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, T h2, T hinv, T h2inv,
+		T h3inv, T& fx, T& fy, T& fz, T& pt) {
+	const T dx = T(int32_t(sx - X));
+	const T dy = T(int32_t(sy - Y));
+	const T dz = T(int32_t(sz - Z));
+	const T r2 = ctfmm::ct_fma(dx, dx, ctfmm::ct_fma(dy, dy, dz * dz));
+	T rinv = ctfmm::ct_rsqrt(r2);
+	T r3 = rinv * rinv * rinv;
 	if (r2 <= h2) {
-		const float q2 = r2 * h2inv;
-		float f = 315.0f / 128.0f;
-		f = __builtin_fmaf(f, q2, -385.0f / 32.0f);
-		f = __builtin_fmaf(f, q2, 1485.0f / 64.0f);
-		f = __builtin_fmaf(f, q2, -693.0f / 32.0f);
-		f = __builtin_fmaf(f, q2, 1155.0f / 128.0f);
-		float p = -63.0f / 256.0f;
-		p = __builtin_fmaf(p, q2, 385.0f / 256.0f);
-		p = __builtin_fmaf(p, q2, -495.0f / 128.0f);
-		p = __builtin_fmaf(p, q2, 693.0f / 128.0f);
-		p = __builtin_fmaf(p, q2, -1155.0f / 256.0f);
-		p = __builtin_fmaf(p, q2, 693.0f / 256.0f);
+		const T q2 = r2 * h2inv;
+		T f = T(315.0 / 128.0);
+		f = ctfmm::ct_fma(f, q2, T(-385.0 / 32.0));
+		f = ctfmm::ct_fma(f, q2, T(1485.0 / 64.0));
+		f = ctfmm::ct_fma(f, q2, T(-693.0 / 32.0));
+		f = ctfmm::ct_fma(f, q2, T(1155.0 / 128.0));
+		T p = T(-63.0 / 256.0);
+		p = ctfmm::ct_fma(p, q2, T(385.0 / 256.0));
+		p = ctfmm::ct_fma(p, q2, T(-495.0 / 128.0));
+		p = ctfmm::ct_fma(p, q2, T(693.0 / 128.0));
+		p = ctfmm::ct_fma(p, q2, T(-1155.0 / 256.0));
+		p = ctfmm::ct_fma(p, q2, T(693.0 / 256.0));
 		r3 = f * h3inv;
 		rinv = p * hinv;
 	}
-	fx = __builtin_fmaf(dx, r3, fx);
-	fy = __builtin_fmaf(dy, r3, fy);
-	fz = __builtin_fmaf(dz, r3, fz);
+	fx = ctfmm::ct_fma(dx, r3, fx);
+	fy = ctfmm::ct_fma(dy, r3, fy);
+	fz = ctfmm::ct_fma(dz, r3, fz);
 	pt -= rinv;
 }
 
 // The same pair without the softening test, for source leaves that cannot
 // come within h of any sink (no pair of the two leaves has r <= h).
-__device__ __forceinline__ void pp_pair_far(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, float& fx, float& fy,
-		float& fz, float& pt) {
-	const float dx = float(int32_t(sx - X));
-	const float dy = float(int32_t(sy - Y));
-	const float dz = float(int32_t(sz - Z));
-	const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-	const float rinv = __builtin_amdgcn_rsqf(r2);
-	const float r3 = rinv * rinv * rinv;
-	fx = __builtin_fmaf(dx, r3, fx);
-	fy = __builtin_fmaf(dy, r3, fy);
-	fz = __builtin_fmaf(dz, r3, fz);
+template<class T>
+__device__ __forceinline__ void pp_pair_far(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, T& fx, T& fy, T& fz,
+		T& pt) {
+	const T dx = T(int32_t(sx - X));
+	const T dy = T(int32_t(sy - Y));
+	const T dz = T(int32_t(sz - Z));
+	const T r2 = ctfmm::ct_fma(dx, dx, ctfmm::ct_fma(dy, dy, dz * dz));
+	const T rinv = ctfmm::ct_rsqrt(r2);
+	const T r3 = rinv * rinv * rinv;
+	fx = ctfmm::ct_fma(dx, r3, fx);
+	fy = ctfmm::ct_fma(dy, r3, fy);
+	fz = ctfmm::ct_fma(dz, r3, fz);
 	pt -= rinv;
+}
+
+// Adds FP32/FP64 tile sums in integer units to the sink accumulators in box
+// units (force 2^64, potential 2^32) and clears the tile sums.
+template<class Acc, class T>
+__device__ __forceinline__ void fold_pp(Acc& gx, Acc& gy, Acc& gz, Acc& ph, T& fx, T& fy, T& fz, T& pt) {
+	gx -= Acc(fx) * Acc(18446744073709551616.0);
+	gy -= Acc(fy) * Acc(18446744073709551616.0);
+	gz -= Acc(fz) * Acc(18446744073709551616.0);
+	ph += Acc(pt) * Acc(4294967296.0);
+	fx = fy = fz = pt = T(0);
 }
 
 // P2P (src/gravity.cu:246-404): one wave per leaf.  Lane (slot, phase) holds
@@ -803,8 +879,10 @@ __device__ __forceinline__ void pp_pair_far(uint32_t sx, uint32_t sy, uint32_t s
 // different banks).  Each tile is summed in FP32, then added to FP64 sink
 // accumulators (compensated at tile granularity); the phases are summed at the
 // end and the result is stored per particle for the kick kernel.
-template<int G, int P>
+template<int G, int P, int PR>
 __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
+	using T = typename Prec<PR>::T;
+	using Acc = typename Prec<PR>::A;
 	constexpr int S = 64 / P;
 	constexpr int STR = kPPTile / P + 16;
 	__shared__ __align__(16) uint32_t tx[P * STR];
@@ -816,8 +894,8 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const int lane = threadIdx.x;
 	const int slot = lane % S, phase = lane / S;
 	uint32_t sx[G], sy[G], sz[G];
-	float fx[G], fy[G], fz[G], pt[G];
-	double gx[G], gy[G], gz[G], ph[G];
+	T fx[G], fy[G], fz[G], pt[G];
+	Acc gx[G], gy[G], gz[G], ph[G];
 	const float hf = A.h;
 #pragma unroll
 	for (int g = 0; g < G; g++) {
@@ -826,16 +904,16 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 		sx[g] = A.px[pbeg + k];
 		sy[g] = A.py[pbeg + k];
 		sz[g] = A.pz[pbeg + k];
-		fx[g] = fy[g] = fz[g] = pt[g] = 0.0f;
-		gx[g] = gy[g] = gz[g] = 0.0;
-		ph[g] = phase == 0 ? double(kSelfPhi / hf) : 0.0;
+		fx[g] = fy[g] = fz[g] = pt[g] = T(0);
+		gx[g] = gy[g] = gz[g] = Acc(0);
+		ph[g] = phase == 0 ? Acc(PR == kF64 ? double(kSelfPhi) / double(hf) : double(kSelfPhi / hf)) : Acc(0);
 	}
 	// integer units: h in units of 2^-32
-	const float hi_ = hf * 4294967296.0f;
-	const float h2 = hi_ * hi_;
-	const float ihinv = 1.0f / hi_;
-	const float ih2inv = ihinv * ihinv;
-	const float ih3inv = ih2inv * ihinv;
+	const T hi_ = T(hf) * T(4294967296.0);
+	const T h2 = hi_ * hi_;
+	const T ihinv = T(1) / hi_;
+	const T ih2inv = ihinv * ihinv;
+	const T ih3inv = ih2inv * ihinv;
 	unsigned long long nsrc = 0;
 	int tn = 0;
 	const uint32_t* bx = tx + phase * STR;
@@ -851,7 +929,7 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 		// not grow with G (LDS reads are ~4% of the VALU cycles)
 #pragma unroll
 		for (int g = 0; g < G; g++) {
-			float ax = fx[g], ay = fy[g], az = fz[g], ap = pt[g];
+			T ax = fx[g], ay = fy[g], az = fz[g], ap = pt[g];
 			const uint32_t X0 = sx[g], Y0 = sy[g], Z0 = sz[g];
 			for (int t = 0; t < q; t += 4) {
 				const uint4 X4 = *reinterpret_cast<const uint4*>(bx + t);
@@ -883,14 +961,12 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			fz[g] = az;
 			pt[g] = ap;
 		}
-		// integer units -> box units: force 2^64, potential 2^32
+		// tile sums into the sink accumulators (all but the uncompensated mode)
+		if constexpr (PR != kF32U) {
 #pragma unroll
-		for (int g = 0; g < G; g++) {
-			gx[g] -= double(fx[g]) * 18446744073709551616.0;
-			gy[g] -= double(fy[g]) * 18446744073709551616.0;
-			gz[g] -= double(fz[g]) * 18446744073709551616.0;
-			ph[g] += double(pt[g]) * 4294967296.0;
-			fx[g] = fy[g] = fz[g] = pt[g] = 0.0f;
+			for (int g = 0; g < G; g++) {
+				fold_pp(gx[g], gy[g], gz[g], ph[g], fx[g], fy[g], fz[g], pt[g]);
+			}
 		}
 		__syncthreads();
 		tn = 0;
@@ -960,6 +1036,12 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			}
 		}
 	}
+	if constexpr (PR == kF32U) {
+#pragma unroll
+		for (int g = 0; g < G; g++) {
+			fold_pp(gx[g], gy[g], gz[g], ph[g], fx[g], fy[g], fz[g], pt[g]);
+		}
+	}
 #pragma unroll
 	for (int g = 0; g < G; g++) {
 		for (int o = S; o < 64; o <<= 1) {
@@ -970,7 +1052,11 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 		}
 		const int k = g * S + slot;
 		if (phase == 0 && k < nsink) {
-			A.pp[pbeg + k] = make_float4(float(gx[g]), float(gy[g]), float(gz[g]), float(ph[g]));
+			if constexpr (PR == kF64) {
+				static_cast<double4*>(A.pp)[pbeg + k] = make_double4(gx[g], gy[g], gz[g], ph[g]);
+			} else {
+				static_cast<float4*>(A.pp)[pbeg + k] = make_float4(float(gx[g]), float(gy[g]), float(gz[g]), float(ph[g]));
+			}
 		}
 	}
 	if (lane == 0) atomicAdd(&A.cnt->stat[0], (unsigned long long) nsrc * (unsigned long long) nsink);
@@ -981,8 +1067,12 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 // pair t on lane t mod 64 with sink t mod nsink, so every lane is busy
 // whatever the leaf size; the 4 results of a pair are added to the sink's FP64
 // sums in LDS.  L2P and the kick then run a sink per lane.
+template<int PR>
 __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
-	__shared__ double acc[4 * kMaxLeafSinks];
+	using T = typename Prec<PR>::T;
+	using Acc = typename Prec<PR>::A;
+	__shared__ Acc acc[4 * kMaxLeafSinks];
+	const T* multi = static_cast<const T*>(A.multi);
 	const int li = blockIdx.x;
 	const int self = uniform(A.leaf_ids[li]);
 	const DNode s = A.nodes[self];
@@ -993,36 +1083,36 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 	const int hpc = uniform(A.head[int64_t(kPC) * A.nnodes + self]);
 	unsigned long long npc = 0;
 	if (hpc >= 0) {
-		for (int k = lane; k < 4 * nsink; k += 64) acc[k] = 0.0;
+		for (int k = lane; k < 4 * nsink; k += 64) acc[k] = Acc(0);
 		__syncthreads();
 		for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
 			npc += sl;
-			const int T = nsink * sl;
-			for (int t = lane; t < T; t += 64) {
+			const int npair = nsink * sl;
+			for (int t = lane; t < npair; t += 64) {
 				const int j = t / nsink;
 				const int k = t - j * nsink;
 				const int pi = pbeg + k;
 				const int o = A.out_pc[sb + j];
-				float M[37];
+				T M[37];
 				uint32_t op[3];
-				load_multi(A.multi + int64_t(o) * kMultiStride, M, op);
-				float D[50];
-				ctfmm::greens<float>(D, fdist(A.px[pi], op[0]), fdist(A.py[pi], op[1]), fdist(A.pz[pi], op[2]));
-				float L4[4] = {0.f, 0.f, 0.f, 0.f};
-				ctfmm::M2L2<float>(L4, M, D, P_.do_phi);
-				atomicAdd(&acc[4 * k + 0], -double(1e6f * L4[1]));
-				atomicAdd(&acc[4 * k + 1], -double(1e6f * L4[2]));
-				atomicAdd(&acc[4 * k + 2], -double(1e6f * L4[3]));
-				atomicAdd(&acc[4 * k + 3], double(1e3f * L4[0]));
+				load_multi(multi + int64_t(o) * kMultiStride, M, op);
+				T D[50];
+				ctfmm::greens<T>(D, tdist<T>(A.px[pi], op[0]), tdist<T>(A.py[pi], op[1]), tdist<T>(A.pz[pi], op[2]));
+				T L4[4] = {T(0), T(0), T(0), T(0)};
+				ctfmm::M2L2<T>(L4, M, D, P_.do_phi);
+				atomicAdd(&acc[4 * k + 0], -Acc(T(1e6) * L4[1]));
+				atomicAdd(&acc[4 * k + 1], -Acc(T(1e6) * L4[2]));
+				atomicAdd(&acc[4 * k + 2], -Acc(T(1e6) * L4[3]));
+				atomicAdd(&acc[4 * k + 3], Acc(T(1e3) * L4[0]));
 			}
 		}
 		__syncthreads();
 	}
 	// ---- L2P + kick, a sink per lane
-	float Ls[50];
+	T Ls[50];
 	{
-		const float* Lp = A.L + int64_t(self) * kLStride;
+		const T* Lp = static_cast<const T*>(A.L) + int64_t(self) * kLStride;
 #pragma unroll
 		for (int q = 0; q < 50; q++) Ls[q] = Lp[q];
 	}
@@ -1032,8 +1122,20 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 	const float sgn = P_.sign * (P_.top ? 1.f : -1.f);
 	for (int k = lane; k < nsink; k += 64) {
 		const int pi = pbeg + k;
-		const float4 pp = A.pp[pi];
-		double gxd = double(pp.x), gyd = double(pp.y), gzd = double(pp.z), phd = double(pp.w);
+		Acc gxd, gyd, gzd, phd;
+		if constexpr (PR == kF64) {
+			const double4 pp = static_cast<const double4*>(A.pp)[pi];
+			gxd = pp.x;
+			gyd = pp.y;
+			gzd = pp.z;
+			phd = pp.w;
+		} else {
+			const float4 pp = static_cast<const float4*>(A.pp)[pi];
+			gxd = Acc(pp.x);
+			gyd = Acc(pp.y);
+			gzd = Acc(pp.z);
+			phd = Acc(pp.w);
+		}
 		if (hpc >= 0) {
 			gxd += acc[4 * k + 0];
 			gyd += acc[4 * k + 1];
@@ -1041,16 +1143,12 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 			phd += acc[4 * k + 3];
 		}
 		const uint32_t X = A.px[pi], Y = A.py[pi], Z = A.pz[pi];
-		float L4[4];
-		ctfmm::L2P<float>(L4, Ls, fdist(X, s.pos[0]), fdist(Y, s.pos[1]), fdist(Z, s.pos[2]), P_.do_phi);
-		float Fphi = float(phd) + 1e3f * L4[0];
-		float Fx = float(gxd) - 1e6f * L4[1];
-		float Fy = float(gyd) - 1e6f * L4[2];
-		float Fz = float(gzd) - 1e6f * L4[3];
-		Fx *= P_.GM;
-		Fy *= P_.GM;
-		Fz *= P_.GM;
-		Fphi *= P_.GM;
+		T L4[4];
+		ctfmm::L2P<T>(L4, Ls, tdist<T>(X, s.pos[0]), tdist<T>(Y, s.pos[1]), tdist<T>(Z, s.pos[2]), P_.do_phi);
+		const float Fphi = float((T(phd) + T(1e3) * L4[0]) * T(P_.GM));
+		const float Fx = float((T(gxd) - T(1e6) * L4[1]) * T(P_.GM));
+		const float Fy = float((T(gyd) - T(1e6) * L4[2]) * T(P_.GM));
+		const float Fz = float((T(gzd) - T(1e6) * L4[3]) * T(P_.GM));
 		float vx = A.vx[pi], vy = A.vy[pi], vz = A.vz[pi];
 		const float kin0 = 0.5f * (vx * vx + vy * vy + vz * vz);
 		float dt;
@@ -1246,6 +1344,7 @@ thread_local std::string g_err;
 
 struct EwaldHost {
 	std::vector<float> rimg, fh, fc;
+	std::vector<double> fc64;  // the Fourier coefficients in double (FP64 kicks)
 	int nreal = 0, nfour = 0;
 };
 
@@ -1290,7 +1389,7 @@ EwaldHost make_ewald_tables() {
 		for (int d = 0; d < 3; d++) E.fh.push_back(float(h[d]));
 		const double h2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
 		const double c = std::exp(-M_PI * M_PI * h2 / (a * a)) / (M_PI * h2);
-		std::vector<float> row(50, 0.0f);
+		std::vector<double> row(50, 0.0);
 		for (int l = 0; l < 7; l++)
 			for (int p = l; p >= 0; p--)
 				for (int q = l - p; q >= 0; q--) {
@@ -1298,9 +1397,10 @@ EwaldHost make_ewald_tables() {
 					if (!(r <= 1 || (p == 0 && q == 0 && r == 2))) continue;
 					const int idx = (p + q) * (p + q + 1) / 2 + q + 28 * (r == 1) + 49 * (r == 2);
 					const double hn = std::pow(double(h[0]), p) * std::pow(double(h[1]), q) * std::pow(double(h[2]), r);
-					row[idx] = float(-2.0 * c * std::pow(2.0 * M_PI, l) * hn * sgn4[l & 3]);
+					row[idx] = -2.0 * c * std::pow(2.0 * M_PI, l) * hn * sgn4[l & 3];
 				}
-		E.fc.insert(E.fc.end(), row.begin(), row.end());
+		for (double v : row) E.fc.push_back(float(v));
+		E.fc64.insert(E.fc64.end(), row.begin(), row.end());
 	}
 	return E;
 }
@@ -1318,6 +1418,7 @@ int upload_ewald() {
 	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rimg), E.rimg.data(), E.rimg.size() * sizeof(float)));
 	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fh), E.fh.data(), E.fh.size() * sizeof(float)));
 	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fc), E.fc.data(), E.fc.size() * sizeof(float)));
+	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_fc64), E.fc64.data(), E.fc64.size() * sizeof(double)));
 	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_nreal), &E.nreal, sizeof(int)));
 	CT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_nfour), &E.nfour, sizeof(int)));
 	float rdt[16];
@@ -1384,6 +1485,108 @@ int download_particles(ct_box* b) {
 	CT_HIP(hipMemcpy(b->hp.vz.data(), b->vz.p, n * 4, hipMemcpyDeviceToHost));
 	CT_HIP(hipMemcpy(b->hp.rung.data(), b->rung.p, n, hipMemcpyDeviceToHost));
 	b->host_stale = false;
+	return CT_OK;
+}
+
+// The interaction phases of one kick in arithmetic mode PR: cells (Ewald,
+// M2L, P2L), the L2L sweep, P2P per leaf class, then M2P + L2P + kick.
+template<int PR>
+static void launch_interactions(ct_box* b, hipStream_t st, const ct_kick_params* p) {
+	using T = typename Prec<PR>::T;
+	const void* multi = PR == kF64 ? static_cast<const void*>(b->multi64.p) : static_cast<const void*>(b->multi.p);
+	void* L = PR == kF64 ? static_cast<void*>(b->L64.p) : static_cast<void*>(b->L.p);
+	CellArgs ca;
+	ca.nodes = b->nodes.p;
+	ca.multi = multi;
+	ca.L = L;
+	ca.head = b->head.p;
+	ca.seg = b->seg.p;
+	ca.out_e = b->out[kEwald].p;
+	ca.out_cc = b->out[kCC].p;
+	ca.out_cp = b->out[kCP].p;
+	ca.px = b->x.p;
+	ca.py = b->y.p;
+	ca.pz = b->z.p;
+	ca.nnodes = int32_t(b->nnodes);
+	ca.do_phi = p->do_phi;
+	ca.cnt = b->cnt.p;
+	const int nb = int(std::min<int64_t>(b->nnodes, 1 << 15));
+	hipLaunchKernelGGL((k_cells_ewald<T>), dim3(nb), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL((k_cells_direct<T>), dim3(nb), dim3(64), 0, st, ca);
+	(void) hipEventRecord(b->ev[2], st);
+	for (int d = 1; d <= b->tree.max_depth; d++) {
+		const int lb = b->tree.level_off[d], le = b->tree.level_off[d + 1];
+		if (le > lb)
+			hipLaunchKernelGGL((k_l2l<T>), dim3((le - lb + 63) / 64), dim3(64), 0, st, b->nodes.p, static_cast<T*>(L), lb, le, p->do_phi);
+	}
+	(void) hipEventRecord(b->ev[3], st);
+	PPArgs pa;
+	pa.nodes = b->nodes.p;
+	pa.head = b->head.p;
+	pa.seg = b->seg.p;
+	pa.out_pp = b->out[kPP].p;
+	pa.nnodes = int32_t(b->nnodes);
+	pa.px = b->x.p;
+	pa.py = b->y.p;
+	pa.pz = b->z.p;
+	pa.pp = b->pp.p;
+	pa.h = p->h;
+	pa.cnt = b->cnt.p;
+	for (int c = 0; c < kNumLeafClasses; c++) {
+		const int64_t lo = b->leaf_class_off[c], cnt = b->leaf_class_off[c + 1] - lo;
+		if (cnt == 0) continue;
+		pa.leaf_ids = b->leaf_ids.p + lo;
+		switch (c) {
+#define CT_LAUNCH_PP(C, G, P) \
+	case C: hipLaunchKernelGGL((k_pp<G, P, PR>), dim3(cnt), dim3(64), 0, st, pa); break;
+			CT_LEAF_CLASSES(CT_LAUNCH_PP)
+#undef CT_LAUNCH_PP
+		}
+	}
+	(void) hipEventRecord(b->ev[6], st);
+	LeafArgs la;
+	la.nodes = b->nodes.p;
+	la.multi = multi;
+	la.L = L;
+	la.head = b->head.p;
+	la.seg = b->seg.p;
+	la.out_pc = b->out[kPC].p;
+	la.nnodes = int32_t(b->nnodes);
+	la.px = b->x.p;
+	la.py = b->y.p;
+	la.pz = b->z.p;
+	la.pp = b->pp.p;
+	la.vx = b->vx.p;
+	la.vy = b->vy.p;
+	la.vz = b->vz.p;
+	la.rung = b->rung.p;
+	la.gx = b->gx.p;
+	la.gy = b->gy.p;
+	la.gz = b->gz.p;
+	la.phi = b->phi.p;
+	la.prm = *p;
+	la.cnt = b->cnt.p;
+	la.leaf_ids = b->leaf_ids.p;
+	la.part = b->part.p;
+	if (b->nleaves) hipLaunchKernelGGL((k_kick<PR>), dim3(b->nleaves), dim3(64), 0, st, la);
+}
+
+// FP64 copies of the multipoles and the FP64 local expansions, made on the
+// first FP64 kick after a tree build.
+static int ensure_fp64(ct_box* b) {
+	const int64_t nn = b->nnodes;
+	int rc;
+	if ((rc = b->L64.grow(std::max<int64_t>(1, nn) * kLStride))) return rc;
+	if (b->multi64_ready) return CT_OK;
+	if ((rc = b->multi64.grow(std::max<int64_t>(1, nn) * kMultiStride))) return rc;
+	std::vector<double> mp(nn * kMultiStride, 0.0);
+	for (int64_t i = 0; i < nn; i++) {
+		const HostNode& h = b->tree.nodes[i];
+		for (int q = 0; q < 37; q++) mp[i * kMultiStride + q] = h.multid[q];
+		for (int q = 0; q < 3; q++) mp[i * kMultiStride + 37 + q] = double(h.pos[q]);
+	}
+	if (nn) CT_HIP(hipMemcpy(b->multi64.p, mp.data(), mp.size() * sizeof(double), hipMemcpyHostToDevice));
+	b->multi64_ready = true;
 	return CT_OK;
 }
 
@@ -1466,6 +1669,9 @@ int ct_box_destroy(ct_box_t b) {
 	b->nodes.release();
 	b->multi.release();
 	b->L.release();
+	b->multi64.release();
+	b->L64.release();
+	b->pp.release();
 	b->leaf_ids.release();
 	for (int q = 0; q < 2; q++) {
 		b->act[q].release();
@@ -1545,6 +1751,7 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 	if ((rc = upload_box(b))) return rc;
 	b->have_tree = true;
 	b->have_lists = false;
+	b->multi64_ready = false;
 	// initial list capacities; grown from the counters on overflow
 	if (b->cap_act == 0) {
 		b->cap_act = std::max<int64_t>(4 * nn, 1024);
@@ -1628,88 +1835,32 @@ static int run_walk(ct_box* b, hipStream_t st, const ct_kick_params* p) {
 }
 
 int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stream) {
+	return ct_kick_prec(b, p, CT_PREC_F32C, ret, stream);
+}
+
+int ct_kick_prec(ct_box_t b, const ct_kick_params* p, int precision, ct_kick_return* ret, void* stream) {
 	int rc = check_init();
 	if (rc) return rc;
 	if (!b || !p) return fail(CT_E_ARG, "ct_kick: bad arguments");
+	if (precision != CT_PREC_F32C && precision != CT_PREC_F32U && precision != CT_PREC_F64) return fail(CT_E_ARG, "ct_kick: bad precision");
 	if (!b->have_tree) return fail(CT_E_STATE, "ct_kick: no tree (call ct_tree_build)");
 	if (p->min_rung < 0 || p->min_rung >= 16 || !(p->theta > 0.0f) || !(p->h > 0.0f)) return fail(CT_E_ARG, "ct_kick: bad params");
+	if (precision == CT_PREC_F64 && (rc = ensure_fp64(b))) return rc;
 	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 	CT_HIP(hipEventRecord(b->ev[0], st));
 	if ((rc = run_walk(b, st, p))) return rc;
 	CT_HIP(hipEventRecord(b->ev[1], st));
-	CellArgs ca;
-	ca.nodes = b->nodes.p;
-	ca.multi = b->multi.p;
-	ca.L = b->L.p;
-	ca.head = b->head.p;
-	ca.seg = b->seg.p;
-	ca.out_e = b->out[kEwald].p;
-	ca.out_cc = b->out[kCC].p;
-	ca.out_cp = b->out[kCP].p;
-	ca.px = b->x.p;
-	ca.py = b->y.p;
-	ca.pz = b->z.p;
-	ca.nnodes = int32_t(b->nnodes);
-	ca.do_phi = p->do_phi;
-	ca.cnt = b->cnt.p;
-	const int nb = int(std::min<int64_t>(b->nnodes, 1 << 15));
-	hipLaunchKernelGGL(k_cells_ewald, dim3(nb), dim3(64), 0, st, ca);
-	hipLaunchKernelGGL(k_cells_direct, dim3(nb), dim3(64), 0, st, ca);
-	CT_HIP(hipEventRecord(b->ev[2], st));
-	for (int d = 1; d <= b->tree.max_depth; d++) {
-		const int lb = b->tree.level_off[d], le = b->tree.level_off[d + 1];
-		if (le > lb) hipLaunchKernelGGL(k_l2l, dim3((le - lb + 63) / 64), dim3(64), 0, st, b->nodes.p, b->L.p, lb, le, p->do_phi);
+	switch (precision) {
+	case CT_PREC_F32U:
+		launch_interactions<kF32U>(b, st, p);
+		break;
+	case CT_PREC_F64:
+		launch_interactions<kF64>(b, st, p);
+		break;
+	default:
+		launch_interactions<kF32C>(b, st, p);
+		break;
 	}
-	CT_HIP(hipEventRecord(b->ev[3], st));
-	PPArgs pa;
-	pa.nodes = b->nodes.p;
-	pa.head = b->head.p;
-	pa.seg = b->seg.p;
-	pa.out_pp = b->out[kPP].p;
-	pa.nnodes = int32_t(b->nnodes);
-	pa.px = b->x.p;
-	pa.py = b->y.p;
-	pa.pz = b->z.p;
-	pa.pp = b->pp.p;
-	pa.h = p->h;
-	pa.cnt = b->cnt.p;
-	LeafArgs la;
-	la.nodes = b->nodes.p;
-	la.multi = b->multi.p;
-	la.L = b->L.p;
-	la.head = b->head.p;
-	la.seg = b->seg.p;
-	la.out_pc = b->out[kPC].p;
-	la.nnodes = int32_t(b->nnodes);
-	la.px = b->x.p;
-	la.py = b->y.p;
-	la.pz = b->z.p;
-	la.pp = b->pp.p;
-	la.vx = b->vx.p;
-	la.vy = b->vy.p;
-	la.vz = b->vz.p;
-	la.rung = b->rung.p;
-	la.gx = b->gx.p;
-	la.gy = b->gy.p;
-	la.gz = b->gz.p;
-	la.phi = b->phi.p;
-	la.prm = *p;
-	la.cnt = b->cnt.p;
-	for (int c = 0; c < kNumLeafClasses; c++) {
-		const int64_t lo = b->leaf_class_off[c], cnt = b->leaf_class_off[c + 1] - lo;
-		if (cnt == 0) continue;
-		pa.leaf_ids = b->leaf_ids.p + lo;
-		switch (c) {
-#define CT_LAUNCH_PP(C, G, P) \
-	case C: hipLaunchKernelGGL((k_pp<G, P>), dim3(cnt), dim3(64), 0, st, pa); break;
-			CT_LEAF_CLASSES(CT_LAUNCH_PP)
-#undef CT_LAUNCH_PP
-		}
-	}
-	CT_HIP(hipEventRecord(b->ev[6], st));
-	la.leaf_ids = b->leaf_ids.p;
-	la.part = b->part.p;
-	if (b->nleaves) hipLaunchKernelGGL(k_kick, dim3(b->nleaves), dim3(64), 0, st, la);
 	CT_HIP(hipEventRecord(b->ev[4], st));
 	hipLaunchKernelGGL(k_reduce, dim3(kReduceBlocks), dim3(256), 0, st, b->part.p, int(b->nleaves), b->red.p + 8);
 	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, b->red.p + 8, kReduceBlocks, b->red.p);

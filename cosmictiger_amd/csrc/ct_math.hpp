@@ -27,18 +27,18 @@ __device__ inline double ct_erfc(double x) { return erfc(x); }
 __device__ inline void ct_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
 __device__ inline void ct_sincos(double x, double* s, double* c) { sincos(x, s, c); }
 #else
-inline float ct_fma(float a, float b, float c) { return std::fma(a, b, c); }
-inline double ct_fma(double a, double b, double c) { return std::fma(a, b, c); }
-inline float ct_rsqrt(float x) { return 1.0f / std::sqrt(x); }
-inline double ct_rsqrt(double x) { return 1.0 / std::sqrt(x); }
-inline float ct_sqrt(float x) { return std::sqrt(x); }
-inline double ct_sqrt(double x) { return std::sqrt(x); }
-inline float ct_exp(float x) { return std::exp(x); }
-inline double ct_exp(double x) { return std::exp(x); }
-inline float ct_erfc(float x) { return std::erfc(x); }
-inline double ct_erfc(double x) { return std::erfc(x); }
-inline void ct_sincos(float x, float* s, float* c) { *s = std::sin(x); *c = std::cos(x); }
-inline void ct_sincos(double x, double* s, double* c) { *s = std::sin(x); *c = std::cos(x); }
+CT_HD inline float ct_fma(float a, float b, float c) { return std::fma(a, b, c); }
+CT_HD inline double ct_fma(double a, double b, double c) { return std::fma(a, b, c); }
+CT_HD inline float ct_rsqrt(float x) { return 1.0f / std::sqrt(x); }
+CT_HD inline double ct_rsqrt(double x) { return 1.0 / std::sqrt(x); }
+CT_HD inline float ct_sqrt(float x) { return std::sqrt(x); }
+CT_HD inline double ct_sqrt(double x) { return std::sqrt(x); }
+CT_HD inline float ct_exp(float x) { return std::exp(x); }
+CT_HD inline double ct_exp(double x) { return std::exp(x); }
+CT_HD inline float ct_erfc(float x) { return std::erfc(x); }
+CT_HD inline double ct_erfc(double x) { return std::erfc(x); }
+CT_HD inline void ct_sincos(float x, float* s, float* c) { *s = std::sin(x); *c = std::cos(x); }
+CT_HD inline void ct_sincos(double x, double* s, double* c) { *s = std::sin(x); *c = std::cos(x); }
 #endif
 
 }  // namespace ctfmm

@@ -321,6 +321,7 @@ Built build(Ctx& c, int64_t b, int64_t e, const double gb[3], const double ge[3]
 	n.depth = depth;
 	for (int i = 0; i < 37; i++) {
 		n.multi[i] = float(rc.multi[i]);
+		n.multid[i] = rc.multi[i];
 	}
 	return rc;
 }

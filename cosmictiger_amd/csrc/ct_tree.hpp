@@ -19,6 +19,7 @@ struct HostNode {
 	int32_t leaf;
 	int32_t depth;
 	float multi[37];       // trless order-6 multipole in the scaled frame
+	double multid[37];     // the same in double (FP64 kicks)
 };
 
 struct HostParticles {

@@ -120,6 +120,16 @@ int ct_tree_build_host(int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, int32_t
  * The call is synchronous with respect to the returned kick_return. */
 int ct_kick(ct_box_t box, const ct_kick_params* params, ct_kick_return* ret, void* stream);
 
+/* Arithmetic of the interaction kernels (the walk and its acceptance tests
+ * are the same in all modes). */
+#define CT_PREC_F32C 0 /* FP32, FP64 sink accumulation per source tile (ct_kick) */
+#define CT_PREC_F32U 1 /* FP32, FP32 running sums */
+#define CT_PREC_F64 2  /* FP64 multipoles, expansions, Ewald and P2P */
+
+/* ct_kick in a chosen arithmetic mode, for the precision sweep.  No
+ * counterpart in the reference, whose kick is FP32 only (src/gravity.cu). */
+int ct_kick_prec(ct_box_t box, const ct_kick_params* params, int precision, ct_kick_return* ret, void* stream);
+
 /* Drift particles of rung `rung` by dt at scale factor a (float arguments as
  * the reference narrows them), with optional drag.  Replaces
  * cuda_drift(char, float a, float dt, ...) (cosmictiger/drift.hpp:54,

@@ -15,7 +15,7 @@ def node_map(pnodes, onodes, oroot):
     return m
 
 
-def run_pair(ct, oc, x, y, z, vel, rung, theta, bucket, h, GM, params_kw=None, nthreads=8):
+def run_pair(ct, oc, x, y, z, vel, rung, theta, bucket, h, GM, params_kw=None, nthreads=8, precision=None):
     """Kick the same box on the GPU and in the oracle.  Returns a dict of both
     sides' results, in original particle order where per-particle."""
     n = len(x)
@@ -25,7 +25,7 @@ def run_pair(ct, oc, x, y, z, vel, rung, theta, bucket, h, GM, params_kw=None, n
     p = ct.default_kick_params(theta, h, GM)
     for k, v in params_kw.items():
         setattr(p, k, v)
-    ret = box.kick(p)
+    ret = box.kick(p, precision=precision)
     out = box.read()
     pnodes = box.nodes()
     lists = [box.lists(k) for k in range(5)]

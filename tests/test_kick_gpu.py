@@ -188,3 +188,23 @@ def test_device_direct_ewald_matches_oracle(gpu, oc, kind, n):
     assert rel.max() <= 1e-9
     assert np.max(np.abs(d["phi"] - ref["phi"])) <= 1e-9 * np.max(np.abs(ref["phi"]))
     box.close()
+
+
+@pytest.mark.parametrize("prec,rms_bar", [("F64", 1e-7), ("F32U", 2e-5), ("F32C", 2e-5)])
+@pytest.mark.parametrize("kind,n,theta", [("uniform", 20000, 0.7), ("clustered", 30000, 0.55)])
+def test_kick_precision_modes(gpu, oc, prec, rms_bar, kind, n, theta):
+    """The three arithmetic modes of ct_kick_prec against the FP64 oracle:
+    identical lists (the walk is FP32 in every mode), forces within the
+    mode's bar.  F64 is bounded by the FP32 storage of the multipoles that the
+    oracle's tree and the product's tree round differently (1e-7)."""
+    x, y, z = _box(kind, n, 23)
+    h = gpu.softening(n)
+    mode = {"F64": gpu.PREC_F64, "F32U": gpu.PREC_F32U, "F32C": gpu.PREC_F32C}[prec]
+    res = run_pair(gpu, oc, x, y, z, None, None, theta, 64, h, 1.0 / n, precision=mode)
+    assert compare_lists(res) == 0
+    err, _ = force_errors(res)
+    dphi = res["gpu"]["phi"].astype(np.float64) - res["oracle"]["phi"]
+    print("%s: force rel err rms %.3e max %.3e; phi rms abs %.3e" % (prec, np.sqrt(np.mean(err ** 2)), err.max(),
+                                                                    np.sqrt(np.mean(dphi ** 2))))
+    assert np.sqrt(np.mean(err ** 2)) <= rms_bar
+    assert np.sqrt(np.mean(dphi ** 2)) <= (1e-5 if prec != "F64" else 1e-7)
