@@ -190,13 +190,13 @@ def test_device_direct_ewald_matches_oracle(gpu, oc, kind, n):
     box.close()
 
 
-@pytest.mark.parametrize("prec,rms_bar", [("F64", 1e-7), ("F32U", 2e-5), ("F32C", 2e-5)])
+@pytest.mark.parametrize("prec,rms_bar", [("F64", 2.4e-7), ("F32U", 2e-5), ("F32C", 2e-5)])
 @pytest.mark.parametrize("kind,n,theta", [("uniform", 20000, 0.7), ("clustered", 30000, 0.55)])
 def test_kick_precision_modes(gpu, oc, prec, rms_bar, kind, n, theta):
     """The three arithmetic modes of ct_kick_prec against the FP64 oracle:
     identical lists (the walk is FP32 in every mode), forces within the
-    mode's bar.  F64 is bounded by the FP32 storage of the multipoles that the
-    oracle's tree and the product's tree round differently (1e-7)."""
+    mode's bar.  F64 computes in double but stores g and phi as float, as the
+    particle arrays are FP32: its bar is 2 ulp of FP32 (2.4e-7)."""
     x, y, z = _box(kind, n, 23)
     h = gpu.softening(n)
     mode = {"F64": gpu.PREC_F64, "F32U": gpu.PREC_F32U, "F32C": gpu.PREC_F32C}[prec]
@@ -207,4 +207,7 @@ def test_kick_precision_modes(gpu, oc, prec, rms_bar, kind, n, theta):
     print("%s: force rel err rms %.3e max %.3e; phi rms abs %.3e" % (prec, np.sqrt(np.mean(err ** 2)), err.max(),
                                                                     np.sqrt(np.mean(dphi ** 2))))
     assert np.sqrt(np.mean(err ** 2)) <= rms_bar
-    assert np.sqrt(np.mean(dphi ** 2)) <= (1e-5 if prec != "F64" else 1e-7)
+    if prec == "F64":
+        assert np.sqrt(np.mean(dphi ** 2)) <= 2.4e-7 * np.max(np.abs(res["oracle"]["phi"]))
+    else:
+        assert np.sqrt(np.mean(dphi ** 2)) <= 1e-5  # GM N = 1
