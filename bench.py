@@ -95,6 +95,70 @@ def precision_sweep(ct, box, x, y, z, h, n, args):
                       "sweep": rows}), flush=True)
 
 
+def ensemble(ct, args):
+    """Independent boxes (N log-uniform in [32^3, 128^3], clustered fraction
+    U[0, 0.8], NFW halos as in the single box, max(8, N/4096) halos per box)
+    kicked together, against the same boxes kicked one per launch."""
+    from cosmictiger_amd import autotune, workloads
+    ct.init(0)
+    theta = args.theta
+    specs = autotune.ensemble_specs(args.ensemble_boxes)
+    t0 = time.perf_counter()
+    boxes, hs, ps = [], [], []
+    for n, frac, seed in specs:
+        x, y, z = workloads.nfw_box(n, seed, clustered_fraction=frac, nhalos=max(8, n // 4096))
+        b = ct.Box(x, y, z)
+        h = ct.softening(n)
+        boxes.append(b)
+        hs.append(h)
+        ps.append(ct.default_kick_params(theta, h, 1.0 / n))
+    ntot = sum(n for n, _, _ in specs)
+    print("ensemble: %d boxes, %d particles, setup %.1fs" % (len(boxes), ntot, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    best, tune = autotune.tune_bucket(ct, boxes, [theta] * len(boxes), hs, ps, log=log)
+    steps = max(2, args.steps)
+    # one box per launch sequence (each box's kick is its own set of launches)
+    for b, p in zip(boxes, ps):
+        b.kick(p)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        for b, p in zip(boxes, ps):
+            b.kick(p)
+    t_single = (time.perf_counter() - t1) / steps
+    batch = ct.Batch(boxes)
+    batch.kick(ps)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        batch.kick(ps)
+    t_eager = (time.perf_counter() - t1) / steps
+    batch.kick(ps, graph=True)
+    batch.kick(ps, graph=True)
+    lat = []
+    for _ in range(steps):
+        t2 = time.perf_counter()
+        batch.kick(ps, graph=True)
+        lat.append(time.perf_counter() - t2)
+    t_graph = float(np.mean(lat))
+    st = batch.stats()
+    batch.close()
+    out = {"metric": "ensemble particle-kicks/s (64 boxes in one batched launch sequence vs one box per launch)",
+           "value": ntot / t_graph, "unit": "particle-kicks/s", "n_gpus": 1, "steps": steps, "higher_is_better": True,
+           "data": "synthetic",
+           "config": {"workload": "%d boxes, N log-uniform in [32^3, 128^3] (total %d), clustered fraction U[0, 0.8], theta %.2f, "
+                                  "autotuned bucket %d" % (len(boxes), ntot, theta, best), "boxes": len(boxes), "n_total": ntot,
+                      "theta": theta, "bucket": best},
+           "one_box_per_launch": {"value": ntot / t_single, "ms_per_step": 1e3 * t_single},
+           "batched_eager": {"value": ntot / t_eager, "ms_per_step": 1e3 * t_eager},
+           "batched_graph": {"value": ntot / t_graph, "ms_per_step": 1e3 * t_graph,
+                             "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))}},
+           "speedup_vs_one_box_per_launch": t_single / t_graph,
+           "autotune_ms_per_batched_kick": {str(k): 1e3 * v for k, v in tune.items()},
+           "events": {k: st[k] for k in ("pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs", "nodes", "sinks", "checks")}}
+    for b in boxes:
+        b.close()
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(x, y, z, args, h):
     """Oracle kick on a bounded sample of sinks of the same box (CPU, this host)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -141,6 +205,10 @@ def main():
     ap.add_argument("--precision-sweep", action="store_true",
                     help="instead of the throughput line: FP32 compensated / FP64 / FP32 uncompensated kicks at theta "
                          "0.4, 0.55, 0.7 with their rms and p99 force errors")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="instead of the throughput line: the 64-box ensemble kicked in one batched, graph-captured "
+                         "launch sequence against one box per launch, with the autotuned bucket")
+    ap.add_argument("--ensemble-boxes", type=int, default=64)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pp_kernel_traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -149,6 +217,9 @@ def main():
         sys.exit(2)
 
     import cosmictiger_amd as ct
+    if args.ensemble:
+        ensemble(ct, args)
+        return
     t0 = time.perf_counter()
     x, y, z = make_box(args)
     n = len(x)

@@ -111,6 +111,11 @@ def _declare(L):
     L.ct_lists_read.argtypes = [_P, ctypes.c_int, _P, _P, ctypes.c_int64]
     L.ct_kick_stats.argtypes = [_P, _P, ctypes.c_int]
     L.ct_direct_ewald.argtypes = [_P, ctypes.c_int64, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P]
+    L.ct_batch_create.argtypes = [_P, ctypes.c_int, ctypes.POINTER(_P)]
+    L.ct_batch_destroy.argtypes = [_P]
+    L.ct_batch_kick.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P]
+    L.ct_batch_drift.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
+    L.ct_batch_stats.argtypes = [_P, _P, ctypes.c_int]
     L.ct_fmm_host.argtypes = [ctypes.c_int, _P, _P]
     L.ct_fmm_device.argtypes = [ctypes.c_int, ctypes.c_int64, _P, _P]
     L.ct_last_error.argtypes = []
@@ -118,7 +123,8 @@ def _declare(L):
     for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_host", "ct_kick",
                  "ct_kick_prec",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
-                 "ct_fmm_device", "ct_direct_ewald"):
+                 "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_kick", "ct_batch_drift",
+                 "ct_batch_stats"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -296,9 +302,55 @@ class Box:
     def stats(self):
         out = np.zeros(15)
         _check(lib().ct_kick_stats(self._h, _ptr(out), 15))
-        keys = ["walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "kick_ms", "pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs",
-                "nodes", "sinks", "pp_ms", "leafkick_ms", "checks"]
-        return dict(zip(keys, out.tolist()))
+        return dict(zip(STAT_KEYS, out.tolist()))
+
+
+STAT_KEYS = ["walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "kick_ms", "pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs", "nodes",
+             "sinks", "pp_ms", "leafkick_ms", "checks"]
+
+
+class Batch:
+    """Resident boxes kicked together: each phase of the kick is one launch over
+    all of them (the reference's kick_workitem list, cosmictiger/kick.hpp:188,
+    extended to many boxes).  Optionally replayed from a captured HIP graph."""
+
+    def __init__(self, boxes):
+        self.boxes = list(boxes)
+        arr = (_P * len(self.boxes))(*[b._h for b in self.boxes])
+        h = _P()
+        _check(lib().ct_batch_create(ctypes.cast(arr, _P), len(self.boxes), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ct_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def kick(self, params, precision=PREC_F32C, graph=False, stream=None):
+        """params: one KickParams per box.  Returns the list of KickReturn."""
+        n = len(self.boxes)
+        parr = (KickParams * n)(*params)
+        rets = (KickReturn * n)()
+        _check(lib().ct_batch_kick(self._h, ctypes.cast(parr, _P), int(precision), ctypes.cast(rets, _P), int(bool(graph)), stream))
+        return list(rets)
+
+    def drift(self, rung, a, dt, drag=None, stream=None):
+        n = len(self.boxes)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), (n,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (n,)))
+        dg = None if drag is None else np.ascontiguousarray(np.broadcast_to(np.asarray(drag, np.float64), (n,)))
+        _check(lib().ct_batch_drift(self._h, int(rung), _ptr(a), _ptr(dt), _ptr(dg), stream))
+
+    def stats(self):
+        out = np.zeros(15)
+        _check(lib().ct_batch_stats(self._h, _ptr(out), 15))
+        return dict(zip(STAT_KEYS, out.tolist()))
 
 
 def default_kick_params(theta, h, GM, eta=None, do_phi=True, save_force=True):

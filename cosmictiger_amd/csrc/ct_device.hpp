@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/cosmictiger_amd.h"
+
 namespace ct {
 
 // 64-byte node record read by the walk and the interaction kernels.
@@ -26,12 +28,45 @@ constexpr int kMultiStride = 40;
 // Local expansion accumulator stride (50 floats padded to 52).
 constexpr int kLStride = 52;
 
-// Walk work entry: one sink node with its direct and Ewald check lists.
+// Walk work entry: one sink node of box `box` with its direct and Ewald check
+// lists (node indices local to the box).
 struct __align__(16) Act {
 	int32_t self;
 	int32_t db, dn;  // direct checks in dp[db .. db+dn)
 	int32_t eb, en;  // Ewald checks in ep[eb .. eb+en)
-	int32_t pad[3];
+	int32_t box;     // index into the batch's BoxDesc table
+	int32_t pad[2];
+};
+
+// One box of a batched kick: its resident arrays in HBM and its kick
+// parameters.  Kernels of a batch index boxes through a table of these; node
+// and particle indices stay local to the box.
+struct __align__(16) BoxDesc {
+	const DNode* nodes;
+	const void* multi;  // float (FP32 modes) or double (FP64) records of kMultiStride
+	void* L;            // float or double, kLStride per node
+	uint32_t* x;
+	uint32_t* y;
+	uint32_t* z;
+	float* vx;
+	float* vy;
+	float* vz;
+	int8_t* rung;
+	float* gx;
+	float* gy;
+	float* gz;
+	float* phi;
+	int32_t* head;  // [kNumLists * nnodes] list heads in the batch's arena
+	void* pp;       // P2P result per particle (float4, or double4 in FP64)
+	int32_t nnodes;
+	int32_t nparts;
+	float theta_inv;
+	float h;
+	ct_kick_params prm;
+	double drift_c1;  // dt / a of the batch drift
+	double drag_c;    // 1 / (1 + drag dt)
+	int32_t drift_rung;
+	int32_t do_drag;
 };
 
 // Output list segment: entries out[begin .. begin+len), next segment of the
@@ -57,8 +92,7 @@ struct Counters {
 };
 
 struct WalkArgs {
-	const DNode* nodes;
-	int32_t nnodes;
+	const BoxDesc* boxes;
 	Act* act[2];
 	int32_t* dp[2];
 	int32_t* ep[2];
@@ -67,10 +101,7 @@ struct WalkArgs {
 	int64_t cap_out[kNumLists];
 	Seg* seg;
 	int64_t cap_seg;
-	int32_t* head;  // [kNumLists * nnodes]
 	Counters* cnt;
-	float theta_inv;
-	float h;
 };
 
 }  // namespace ct

@@ -54,6 +54,10 @@ struct DBuf {
 // ct_init has been called (device selected, Ewald tables uploaded).
 int check_init();
 
+// Source of tree versions: every tree build takes a new one, so that kick
+// contexts can tell that a box's tree changed.
+extern uint64_t g_tree_version;
+
 }  // namespace ct
 
 // A resident box: host copy of the particles and tree, device state in HBM.
@@ -64,6 +68,7 @@ struct ct_box {
 	bool host_stale = false;  // device particles are newer than the host copy
 	double theta = 0.7, hsoft = 0.0;
 	int bucket = 128;
+	uint64_t tree_version = 0;
 	// device particle state (tree order)
 	ct::DBuf<uint32_t> x, y, z;
 	ct::DBuf<float> vx, vy, vz, gx, gy, gz, phi;
@@ -71,24 +76,11 @@ struct ct_box {
 	// tree
 	ct::DBuf<ct::DNode> nodes;
 	ct::DBuf<float> multi, L;
-	ct::DBuf<int32_t> leaf_ids;  // non-empty leaves grouped by size class
-	int64_t leaf_class_off[13] = {0};
-	ct::DBuf<double4> pp;        // P2P result per particle (float4 or double4 records)
+	ct::DBuf<double4> pp;           // P2P result per particle (float4 or double4 records)
 	ct::DBuf<double> multi64, L64;  // FP64 multipoles and expansions (FP64 kicks only)
 	bool multi64_ready = false;
-	int64_t nnodes = 0, nleaves = 0;
-	// walk
-	ct::DBuf<ct::Act> act[2];
-	ct::DBuf<int32_t> dp[2], ep[2];
-	ct::DBuf<int32_t> out[ct::kNumLists];
-	ct::DBuf<ct::Seg> seg;
-	ct::DBuf<int32_t> head;
-	ct::DBuf<ct::Counters> cnt;
-	ct::DBuf<double> part, red;
-	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[ct::kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
-	// diagnostics
-	hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-	double stats[15] = {0};
-	ct::Counters last_counters;
-	bool have_lists = false;
+	int64_t nnodes = 0;
+	std::vector<int32_t> leaf_class[16];  // non-empty leaves by size class
+	// kick context of single-box kicks (walk arenas, lists of the last kick)
+	void* self_ctx = nullptr;
 };

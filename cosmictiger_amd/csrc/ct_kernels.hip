@@ -164,15 +164,15 @@ struct WalkCounts {
 	int cc, cp, pc, lf, nx;
 };
 
-__device__ __forceinline__ WalkCounts walk_count(const WalkArgs& A, const DNode& s, const int32_t* dp, const int32_t* ep, int db, int dn, int eb,
-		int en, int lane) {
+__device__ __forceinline__ WalkCounts walk_count(const DNode* __restrict__ nodes, float thinv, float h, const DNode& s, const int32_t* dp,
+		const int32_t* ep, int db, int dn, int eb, int en, int lane) {
 	WalkCounts w = {0, 0, 0, 0, 0, 0, 0, 0};
 	for (int base = 0; base < en; base += 64) {
 		const int i = base + lane;
 		int cat = K_NONE;
 		if (i < en) {
-			const DNode o = A.nodes[ep[eb + i]];
-			cat = classify_ewald(s, o, A.theta_inv);
+			const DNode o = nodes[ep[eb + i]];
+			cat = classify_ewald(s, o, thinv);
 		}
 		w.ecc += __popcll(__ballot(cat == K_CC));
 		w.enext += __popcll(__ballot(cat == K_NEXT));
@@ -182,8 +182,8 @@ __device__ __forceinline__ WalkCounts walk_count(const WalkArgs& A, const DNode&
 		const int i = base + lane;
 		int cat = K_NONE;
 		if (i < dn) {
-			const DNode o = A.nodes[dp[db + i]];
-			cat = classify_direct(s, o, A.theta_inv, A.h);
+			const DNode o = nodes[dp[db + i]];
+			cat = classify_direct(s, o, thinv, h);
 		}
 		w.cc += __popcll(__ballot(cat == K_CC));
 		w.cp += __popcll(__ballot(cat == K_CP));
@@ -233,10 +233,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 		for (int e = 0; e < nb; e++) {
 			const Act ae = act[b0 + e];
 			const int self = uniform(ae.self);
-			const DNode s = A.nodes[self];
+			const BoxDesc* B = A.boxes + uniform(ae.box);
+			const DNode* nodes = B->nodes;
+			const DNode s = nodes[self];
 			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
 			const bool sleaf = s.flags & 1;
-			const WalkCounts w = walk_count(A, s, dp, ep, uniform(ae.db), uniform(ae.dn), uniform(ae.eb), uniform(ae.en), lane);
+			const WalkCounts w = walk_count(nodes, B->theta_inv, B->h, s, dp, ep, uniform(ae.db), uniform(ae.dn), uniform(ae.eb), uniform(ae.en),
+					lane);
 			nchecks += uniform(ae.dn) + uniform(ae.en);
 			if (w.efail || (sleaf && w.enext)) {
 				if (lane == 0) atomicOr(&C->walk_error, 1);
@@ -303,9 +306,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 			if (__builtin_amdgcn_readlane(my_skip, e)) continue;
 			const Act ae = act[b0 + e];
 			const int self = uniform(ae.self);
+			const int box = uniform(ae.box);
+			const BoxDesc* B = A.boxes + box;
+			const DNode* nodes = B->nodes;
+			const float thinv = B->theta_inv, hs = B->h;
 			const int db = uniform(ae.db), dn = uniform(ae.dn);
 			const int eb = uniform(ae.eb), en = uniform(ae.en);
-			const DNode s = A.nodes[self];
+			const DNode s = nodes[self];
 			const bool sleaf = s.flags & 1;
 			int off[R_NUM], cnt[R_NUM];
 #pragma unroll
@@ -330,8 +337,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 					DNode o;
 					if (i < en) {
 						oi = ep[eb + i];
-						o = A.nodes[oi];
-						cat = classify_ewald(s, o, A.theta_inv);
+						o = nodes[oi];
+						cat = classify_ewald(s, o, thinv);
 					}
 					const unsigned long long mcc = __ballot(cat == K_CC);
 					const unsigned long long mnx = __ballot(cat == K_NEXT);
@@ -359,8 +366,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 					DNode o;
 					if (i < dn) {
 						oi = dp[db + i];
-						o = A.nodes[oi];
-						cat = classify_direct(s, o, A.theta_inv, A.h);
+						o = nodes[oi];
+						cat = classify_direct(s, o, thinv, hs);
 					}
 					const unsigned long long mcc = __ballot(cat == K_CC);
 					const unsigned long long mcp = __ballot(cat == K_CP);
@@ -403,7 +410,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 			if (lane == 0) {
 				if (nnext && ok[R_ACT] && ok[R_DP] && ok[R_EP]) {
 					Act en_;
-					en_.pad[0] = en_.pad[1] = en_.pad[2] = 0;
+					en_.box = box;
+					en_.pad[0] = en_.pad[1] = 0;
 					if (sleaf) {
 						en_.self = self;
 						en_.db = r_dp;
@@ -434,7 +442,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 						Seg sg;
 						sg.begin = offs[q];
 						sg.len = cnts[q];
-						int32_t* hp = A.head + int64_t(kinds[q]) * A.nnodes + self;
+						int32_t* hp = B->head + int64_t(kinds[q]) * B->nnodes + self;
 						sg.next = *hp;
 						sg.pad = 0;
 						A.seg[sid] = sg;
@@ -449,13 +457,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 }
 
 struct InitArgs {
-	int32_t* head;
+	int32_t* head;  // the batch's head arena
 	int64_t nhead;
 	Counters* cnt;
 	Act* act0;
 	int32_t* dp0;
 	int32_t* ep0;
-	int32_t root;
+	int32_t nbox;
 };
 
 __global__ void k_walk_init(InitArgs A) {
@@ -469,25 +477,27 @@ __global__ void k_walk_init(InitArgs A) {
 	for (int64_t j = i; j < nc; j += stride) {
 		c[j] = 0;
 	}
-	if (i == 0) {
+	// one entry per box: its root with the check lists {root}, {root}
+	for (int64_t b = i; b < A.nbox; b += stride) {
 		Act e;
-		e.self = A.root;
-		e.db = 0;
+		e.self = 0;
+		e.db = int32_t(b);
 		e.dn = 1;
-		e.eb = 0;
+		e.eb = int32_t(b);
 		e.en = 1;
-		e.pad[0] = e.pad[1] = e.pad[2] = 0;
-		A.act0[0] = e;
-		A.dp0[0] = A.root;
-		A.ep0[0] = A.root;
+		e.box = int32_t(b);
+		e.pad[0] = e.pad[1] = 0;
+		A.act0[b] = e;
+		A.dp0[b] = 0;
+		A.ep0[b] = 0;
 	}
 }
 
-__global__ void k_walk_seed_counts(Counters* C) {
+__global__ void k_walk_seed_counts(Counters* C, int nbox) {
 	if (threadIdx.x == 0 && blockIdx.x == 0) {
-		C->act[0] = 1;
-		C->dp[0] = 1;
-		C->ep[0] = 1;
+		C->act[0] = nbox;
+		C->dp[0] = nbox;
+		C->ep[0] = nbox;
 	}
 }
 
@@ -531,21 +541,30 @@ template<> __device__ __forceinline__ const double* ewald_fc<double>() {
 }
 
 struct CellArgs {
-	const DNode* nodes;
-	const void* multi;  // float or double records of kMultiStride
-	void* L;            // float or double, kLStride per node
-	const int32_t* head;
+	const BoxDesc* boxes;
+	const int32_t* node_off;  // [nbox + 1]: the boxes' nodes in one index space
+	int32_t nbox;
+	int32_t total;  // node_off[nbox]
 	const Seg* seg;
 	const int32_t* out_e;
 	const int32_t* out_cc;
 	const int32_t* out_cp;
-	const uint32_t* px;
-	const uint32_t* py;
-	const uint32_t* pz;
-	int32_t nnodes;
-	bool do_phi;
 	Counters* cnt;
 };
+
+// Box b with off[b] <= g < off[b + 1] (off non-decreasing, off[0] = 0).
+__device__ __forceinline__ int find_box(const int32_t* __restrict__ off, int nbox, int g) {
+	int lo = 0, hi = nbox;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (off[mid] <= g) {
+			lo = mid;
+		} else {
+			hi = mid;
+		}
+	}
+	return lo;
+}
 
 __device__ __forceinline__ void load_multi(const float* __restrict__ mp, float* M, uint32_t* pos) {
 	const float4* m4 = reinterpret_cast<const float4*>(mp);
@@ -625,16 +644,20 @@ constexpr int kCellTile = 512;
 template<class T>
 __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 	const int lane = threadIdx.x;
-	const T* multi = static_cast<const T*>(A.multi);
 	unsigned long long nev = 0;
-	for (int i = blockIdx.x; i < A.nnodes; i += gridDim.x) {
-		T* Lo = static_cast<T*>(A.L) + int64_t(i) * kLStride;
-		const int h0 = uniform(A.head[int64_t(kEwald) * A.nnodes + i]);
+	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
+		const int bx = uniform(find_box(A.node_off, A.nbox, g));
+		const BoxDesc* B = A.boxes + bx;
+		const int i = g - A.node_off[bx];
+		const T* multi = static_cast<const T*>(B->multi);
+		T* Lo = static_cast<T*>(B->L) + int64_t(i) * kLStride;
+		const bool do_phi = B->prm.do_phi;
+		const int h0 = uniform(B->head[int64_t(kEwald) * B->nnodes + i]);
 		if (h0 < 0) {
 			if (lane < 50) Lo[lane] = T(0);
 			continue;
 		}
-		const uint32_t sp0 = uniform(int(A.nodes[i].pos[0])), sp1 = uniform(int(A.nodes[i].pos[1])), sp2 = uniform(int(A.nodes[i].pos[2]));
+		const uint32_t sp0 = uniform(int(B->nodes[i].pos[0])), sp1 = uniform(int(B->nodes[i].pos[1])), sp2 = uniform(int(B->nodes[i].pos[2]));
 		T L[50];
 #pragma unroll
 		for (int q = 0; q < 50; q++) L[q] = T(0);
@@ -648,7 +671,7 @@ __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 				T D[50];
 				ctfmm::ewald_greens<T, T>(D, tdist<T>(sp0, op[0]), tdist<T>(sp1, op[1]), tdist<T>(sp2, op[2]), c_rimg, c_nreal, c_fh, ewald_fc<T>(),
 						c_nfour);
-				ctfmm::M2L<T>(L, M, D, A.do_phi);
+				ctfmm::M2L<T>(L, M, D, do_phi);
 				nev++;
 			}
 		}
@@ -665,13 +688,17 @@ template<class T>
 __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 	__shared__ uint32_t tx[kCellTile], ty[kCellTile], tz[kCellTile];
 	const int lane = threadIdx.x;
-	const T* multi = static_cast<const T*>(A.multi);
 	unsigned long long ncc = 0, ncp = 0;
-	for (int i = blockIdx.x; i < A.nnodes; i += gridDim.x) {
-		const int hcc = uniform(A.head[int64_t(kCC) * A.nnodes + i]);
-		const int hcp = uniform(A.head[int64_t(kCP) * A.nnodes + i]);
+	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
+		const int bx = uniform(find_box(A.node_off, A.nbox, g));
+		const BoxDesc* B = A.boxes + bx;
+		const int i = g - A.node_off[bx];
+		const T* multi = static_cast<const T*>(B->multi);
+		const bool do_phi = B->prm.do_phi;
+		const int hcc = uniform(B->head[int64_t(kCC) * B->nnodes + i]);
+		const int hcp = uniform(B->head[int64_t(kCP) * B->nnodes + i]);
 		if (hcc < 0 && hcp < 0) continue;
-		const uint32_t sp0 = uniform(int(A.nodes[i].pos[0])), sp1 = uniform(int(A.nodes[i].pos[1])), sp2 = uniform(int(A.nodes[i].pos[2]));
+		const uint32_t sp0 = uniform(int(B->nodes[i].pos[0])), sp1 = uniform(int(B->nodes[i].pos[1])), sp2 = uniform(int(B->nodes[i].pos[2]));
 		T L[50];
 #pragma unroll
 		for (int q = 0; q < 50; q++) L[q] = T(0);
@@ -684,7 +711,7 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 				load_multi(multi + int64_t(o) * kMultiStride, M, op);
 				T D[50];
 				ctfmm::greens<T>(D, tdist<T>(sp0, op[0]), tdist<T>(sp1, op[1]), tdist<T>(sp2, op[2]));
-				ctfmm::M2L<T>(L, M, D, A.do_phi);
+				ctfmm::M2L<T>(L, M, D, do_phi);
 				ncc++;
 			}
 		}
@@ -705,14 +732,14 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
 			for (int j = 0; j < sl; j++) {
 				const int o = uniform(A.out_cp[sb + j]);
-				int pb = uniform(A.nodes[o].pbegin);
-				const int pe = uniform(A.nodes[o].pend);
+				int pb = uniform(B->nodes[o].pbegin);
+				const int pe = uniform(B->nodes[o].pend);
 				while (pb < pe) {
 					const int m = min(kCellTile - tn, pe - pb);
 					for (int t = lane; t < m; t += 64) {
-						tx[tn + t] = A.px[pb + t];
-						ty[tn + t] = A.py[pb + t];
-						tz[tn + t] = A.pz[pb + t];
+						tx[tn + t] = B->x[pb + t];
+						ty[tn + t] = B->y[pb + t];
+						tz[tn + t] = B->z[pb + t];
 					}
 					tn += m;
 					pb += m;
@@ -721,7 +748,7 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 			}
 		}
 		if (tn) flush();
-		wave_add_expansion(static_cast<T*>(A.L) + int64_t(i) * kLStride, L, lane, false);
+		wave_add_expansion(static_cast<T*>(B->L) + int64_t(i) * kLStride, L, lane, false);
 	}
 	ncc = wave_sum_u64(ncc);
 	ncp = wave_sum_u64(ncp);
@@ -735,19 +762,33 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 // L2L sweep, one launch per level
 // ---------------------------------------------------------------------------
 
+// Work list of one tree level of every box: entry t is node lo[b] + t - pref[b]
+// of box b, pref[b] <= t < pref[b + 1].
+struct L2LArgs {
+	const BoxDesc* boxes;
+	const int32_t* pref;  // [nbox + 1]
+	const int32_t* lo;    // [nbox]
+	int32_t nbox;
+	int32_t total;
+};
+
 template<class T>
-__global__ __launch_bounds__(64) void k_l2l(const DNode* __restrict__ nodes, T* __restrict__ L, int begin, int end, bool do_phi) {
-	const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= end) return;
-	const DNode s = nodes[i];
+__global__ __launch_bounds__(64) void k_l2l(L2LArgs A) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= A.total) return;
+	const int bx = find_box(A.pref, A.nbox, t);
+	const BoxDesc* B = A.boxes + bx;
+	const int i = A.lo[bx] + t - A.pref[bx];
+	const DNode s = B->nodes[i];
 	if (s.parent < 0) return;
-	const DNode p = nodes[s.parent];
+	const DNode p = B->nodes[s.parent];
+	T* L = static_cast<T*>(B->L);
 	const T* La = L + int64_t(s.parent) * kLStride;
 	T Lp[50];
 #pragma unroll
 	for (int q = 0; q < 50; q++) Lp[q] = La[q];
 	T Lb[50];
-	ctfmm::L2L<T>(Lb, Lp, tdist<T>(s.pos[0], p.pos[0]), tdist<T>(s.pos[1], p.pos[1]), tdist<T>(s.pos[2], p.pos[2]), do_phi);
+	ctfmm::L2L<T>(Lb, Lp, tdist<T>(s.pos[0], p.pos[0]), tdist<T>(s.pos[1], p.pos[1]), tdist<T>(s.pos[2], p.pos[2]), B->prm.do_phi);
 	T* Lo = L + int64_t(i) * kLStride;
 #pragma unroll
 	for (int q = 0; q < 50; q++) Lo[q] += Lb[q];
@@ -769,43 +810,19 @@ __host__ __device__ inline int leaf_class(int n) {
 }
 
 struct PPArgs {
-	const DNode* nodes;
-	const int32_t* leaf_ids;  // leaves of this class
-	const int32_t* head;
+	const BoxDesc* boxes;
+	const int2* leaves;  // (box, leaf) of this class
 	const Seg* seg;
 	const int32_t* out_pp;
-	int32_t nnodes;
-	const uint32_t* px;
-	const uint32_t* py;
-	const uint32_t* pz;
-	void* pp;  // per particle: P2P force (gx, gy, gz) and potential, box units (float4, or double4 in FP64)
-	float h;
 	Counters* cnt;
 };
 
 struct LeafArgs {
-	const DNode* nodes;
-	const int32_t* leaf_ids;  // non-empty leaves
-	const void* multi;
-	const void* L;
-	const int32_t* head;
+	const BoxDesc* boxes;
+	const int2* leaves;  // (box, leaf), grouped by box
 	const Seg* seg;
 	const int32_t* out_pc;
-	int32_t nnodes;
-	const uint32_t* px;
-	const uint32_t* py;
-	const uint32_t* pz;
-	const void* pp;
-	float* vx;
-	float* vy;
-	float* vz;
-	int8_t* rung;
-	float* gx;
-	float* gy;
-	float* gz;
-	float* phi;
-	double* part;  // 8 per leaf
-	ct_kick_params prm;
+	double* part;  // 8 per entry of leaves
 	Counters* cnt;
 };
 
@@ -888,22 +905,28 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	__shared__ __align__(16) uint32_t tx[P * STR];
 	__shared__ __align__(16) uint32_t ty[P * STR];
 	__shared__ __align__(16) uint32_t tz[P * STR];
-	const int self = uniform(A.leaf_ids[blockIdx.x]);
-	const int pbeg = uniform(A.nodes[self].pbegin);
-	const int nsink = uniform(A.nodes[self].pend) - pbeg;
+	const int2 le = A.leaves[blockIdx.x];
+	const BoxDesc* B = A.boxes + uniform(le.x);
+	const int self = uniform(le.y);
+	const DNode* nodes = B->nodes;
+	const uint32_t* px = B->x;
+	const uint32_t* py = B->y;
+	const uint32_t* pz = B->z;
+	const int pbeg = uniform(nodes[self].pbegin);
+	const int nsink = uniform(nodes[self].pend) - pbeg;
 	const int lane = threadIdx.x;
 	const int slot = lane % S, phase = lane / S;
 	uint32_t sx[G], sy[G], sz[G];
 	T fx[G], fy[G], fz[G], pt[G];
 	Acc gx[G], gy[G], gz[G], ph[G];
-	const float hf = A.h;
+	const float hf = B->h;
 #pragma unroll
 	for (int g = 0; g < G; g++) {
 		int k = g * S + slot;
 		if (k >= nsink) k = 0;
-		sx[g] = A.px[pbeg + k];
-		sy[g] = A.py[pbeg + k];
-		sz[g] = A.pz[pbeg + k];
+		sx[g] = px[pbeg + k];
+		sy[g] = py[pbeg + k];
+		sz[g] = pz[pbeg + k];
 		fx[g] = fy[g] = fz[g] = pt[g] = T(0);
 		gx[g] = gy[g] = gz[g] = Acc(0);
 		ph[g] = phase == 0 ? Acc(PR == kF64 ? double(kSelfPhi) / double(hf) : double(kSelfPhi / hf)) : Acc(0);
@@ -975,9 +998,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	// sink leaf's: dist(centres) - r_s - r_o <= h (plus 1e-6 for the float
 	// rounding of the centres, radii and distance).  Pass 0 stages the far
 	// leaves, pass 1 the near ones.
-	const uint32_t cs0 = uniform(int(A.nodes[self].pos[0])), cs1 = uniform(int(A.nodes[self].pos[1])), cs2 = uniform(int(A.nodes[self].pos[2]));
-	const float rs = A.nodes[self].radius;
-	const int hpp = uniform(A.head[int64_t(kPP) * A.nnodes + self]);
+	const uint32_t cs0 = uniform(int(nodes[self].pos[0])), cs1 = uniform(int(nodes[self].pos[1])), cs2 = uniform(int(nodes[self].pos[2]));
+	const float rs = nodes[self].radius;
+	const int hpp = uniform(B->head[int64_t(kPP) * B->nnodes + self]);
 	// stage particles [pb, pe) into the tile, flushing whenever it fills
 	auto stage = [&](int pb, const int pe, bool nearpass) __attribute__((always_inline)) {
 		while (pb < pe) {
@@ -986,9 +1009,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			for (int t = lane; t < m; t += 64) {
 				const int i = tn + t;
 				const int at = (i % P) * STR + i / P;
-				tx[at] = A.px[pb + t];
-				ty[at] = A.py[pb + t];
-				tz[at] = A.pz[pb + t];
+				tx[at] = px[pb + t];
+				ty[at] = py[pb + t];
+				tz[at] = pz[pb + t];
 			}
 			tn += m;
 			pb += m;
@@ -1008,7 +1031,7 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
 			for (int j = 0; j < sl; j++) {
 				const int o = uniform(A.out_pp[sb + j]);
-				const DNode& on = A.nodes[o];
+				const DNode& on = nodes[o];
 				const float ddx = fdist(cs0, uint32_t(uniform(int(on.pos[0]))));
 				const float ddy = fdist(cs1, uint32_t(uniform(int(on.pos[1]))));
 				const float ddz = fdist(cs2, uint32_t(uniform(int(on.pos[2]))));
@@ -1053,9 +1076,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 		const int k = g * S + slot;
 		if (phase == 0 && k < nsink) {
 			if constexpr (PR == kF64) {
-				static_cast<double4*>(A.pp)[pbeg + k] = make_double4(gx[g], gy[g], gz[g], ph[g]);
+				static_cast<double4*>(B->pp)[pbeg + k] = make_double4(gx[g], gy[g], gz[g], ph[g]);
 			} else {
-				static_cast<float4*>(A.pp)[pbeg + k] = make_float4(float(gx[g]), float(gy[g]), float(gz[g]), float(ph[g]));
+				static_cast<float4*>(B->pp)[pbeg + k] = make_float4(float(gx[g]), float(gy[g]), float(gz[g]), float(ph[g]));
 			}
 		}
 	}
@@ -1072,15 +1095,20 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 	using T = typename Prec<PR>::T;
 	using Acc = typename Prec<PR>::A;
 	__shared__ Acc acc[4 * kMaxLeafSinks];
-	const T* multi = static_cast<const T*>(A.multi);
 	const int li = blockIdx.x;
-	const int self = uniform(A.leaf_ids[li]);
-	const DNode s = A.nodes[self];
+	const int2 le = A.leaves[li];
+	const BoxDesc* B = A.boxes + uniform(le.x);
+	const int self = uniform(le.y);
+	const T* multi = static_cast<const T*>(B->multi);
+	const DNode s = B->nodes[self];
 	const int pbeg = s.pbegin;
 	const int nsink = s.pend - s.pbegin;
 	const int lane = threadIdx.x;
-	const ct_kick_params& P_ = A.prm;
-	const int hpc = uniform(A.head[int64_t(kPC) * A.nnodes + self]);
+	const ct_kick_params P_ = B->prm;
+	const uint32_t* px = B->x;
+	const uint32_t* py = B->y;
+	const uint32_t* pz = B->z;
+	const int hpc = uniform(B->head[int64_t(kPC) * B->nnodes + self]);
 	unsigned long long npc = 0;
 	if (hpc >= 0) {
 		for (int k = lane; k < 4 * nsink; k += 64) acc[k] = Acc(0);
@@ -1098,7 +1126,7 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 				uint32_t op[3];
 				load_multi(multi + int64_t(o) * kMultiStride, M, op);
 				T D[50];
-				ctfmm::greens<T>(D, tdist<T>(A.px[pi], op[0]), tdist<T>(A.py[pi], op[1]), tdist<T>(A.pz[pi], op[2]));
+				ctfmm::greens<T>(D, tdist<T>(px[pi], op[0]), tdist<T>(py[pi], op[1]), tdist<T>(pz[pi], op[2]));
 				T L4[4] = {T(0), T(0), T(0), T(0)};
 				ctfmm::M2L2<T>(L4, M, D, P_.do_phi);
 				atomicAdd(&acc[4 * k + 0], -Acc(T(1e6) * L4[1]));
@@ -1112,7 +1140,7 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 	// ---- L2P + kick, a sink per lane
 	T Ls[50];
 	{
-		const T* Lp = static_cast<const T*>(A.L) + int64_t(self) * kLStride;
+		const T* Lp = static_cast<const T*>(B->L) + int64_t(self) * kLStride;
 #pragma unroll
 		for (int q = 0; q < 50; q++) Ls[q] = Lp[q];
 	}
@@ -1124,13 +1152,13 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 		const int pi = pbeg + k;
 		Acc gxd, gyd, gzd, phd;
 		if constexpr (PR == kF64) {
-			const double4 pp = static_cast<const double4*>(A.pp)[pi];
+			const double4 pp = static_cast<const double4*>(B->pp)[pi];
 			gxd = pp.x;
 			gyd = pp.y;
 			gzd = pp.z;
 			phd = pp.w;
 		} else {
-			const float4 pp = static_cast<const float4*>(A.pp)[pi];
+			const float4 pp = static_cast<const float4*>(B->pp)[pi];
 			gxd = Acc(pp.x);
 			gyd = Acc(pp.y);
 			gzd = Acc(pp.z);
@@ -1142,14 +1170,14 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 			gzd += acc[4 * k + 2];
 			phd += acc[4 * k + 3];
 		}
-		const uint32_t X = A.px[pi], Y = A.py[pi], Z = A.pz[pi];
+		const uint32_t X = px[pi], Y = py[pi], Z = pz[pi];
 		T L4[4];
 		ctfmm::L2P<T>(L4, Ls, tdist<T>(X, s.pos[0]), tdist<T>(Y, s.pos[1]), tdist<T>(Z, s.pos[2]), P_.do_phi);
 		const float Fphi = float((T(phd) + T(1e3) * L4[0]) * T(P_.GM));
 		const float Fx = float((T(gxd) - T(1e6) * L4[1]) * T(P_.GM));
 		const float Fy = float((T(gyd) - T(1e6) * L4[2]) * T(P_.GM));
 		const float Fz = float((T(gzd) - T(1e6) * L4[3]) * T(P_.GM));
-		float vx = A.vx[pi], vy = A.vy[pi], vz = A.vz[pi];
+		float vx = B->vx[pi], vy = B->vy[pi], vz = B->vz[pi];
 		const float kin0 = 0.5f * (vx * vx + vy * vy + vz * vz);
 		float dt;
 		if (P_.ascending) {
@@ -1166,18 +1194,18 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 		zm += vz;
 		nm += sqrtf(vx * vx + vy * vy + vz * vz);
 		if (P_.save_force) {
-			A.gx[pi] = Fx;
-			A.gy[pi] = Fy;
-			A.gz[pi] = Fz;
-			A.phi[pi] = Fphi;
+			B->gx[pi] = Fx;
+			B->gy[pi] = Fy;
+			B->gz[pi] = Fz;
+			B->phi[pi] = Fphi;
 		}
 		if (P_.descending) {
 			const float g2 = Fx * Fx + Fy * Fy + Fz * Fz;
 			dt = fminf(P_.eta * sqrtf(P_.a * hf * rsqrtf(g2)), P_.t0);
 			dt = fminf(P_.max_dt, dt);
-			int r = A.rung[pi];
+			int r = B->rung[pi];
 			r = max(P_.min_rung + int(int(ceilf(log2f(P_.t0 / dt))) > P_.min_rung), r - 1);
-			A.rung[pi] = int8_t(r);
+			B->rung[pi] = int8_t(r);
 			maxr = max(maxr, r);
 			dt = 0.5f * c_rung_dt[P_.min_rung] * P_.t0;
 			vx = __builtin_fmaf(sgn * Fx, dt, vx);
@@ -1185,9 +1213,9 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 			vz = __builtin_fmaf(sgn * Fz, dt, vz);
 		}
 		const float kin1 = 0.5f * (vx * vx + vy * vy + vz * vz);
-		A.vx[pi] = vx;
-		A.vy[pi] = vy;
-		A.vz[pi] = vz;
+		B->vx[pi] = vx;
+		B->vy[pi] = vy;
+		B->vz[pi] = vz;
 		pot += 0.5 * double(Fphi);
 		dkin += double(kin1) - double(kin0);
 	}
@@ -1220,18 +1248,16 @@ __global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
 #define CT_LEAF_CLASSES(X) X(0, 1, 4) X(1, 2, 4) X(2, 3, 4) X(3, 4, 4) X(4, 5, 4) X(5, 6, 4) \
 	X(6, 4, 2) X(7, 5, 2) X(8, 6, 2) X(9, 7, 2) X(10, 8, 2)
 
-// Deterministic FP64 reduction of the per-leaf kick_return partials: block b
-// reduces the b-th contiguous slice of the n records (8 doubles each: max_rung
-// by max, the rest by sum) into out[8 b ..]; a second launch with one block
-// reduces those.
-constexpr int kReduceBlocks = 256;
+// Deterministic FP64 reduction of the per-leaf kick_return partials (8
+// doubles per leaf: max_rung by max, the rest by sum).  Stage 1: block c
+// reduces the records [r.x, r.y) of chunk c into out[8 c ..]; chunks never
+// cross a box boundary.  Stage 2: block b reduces the chunk results
+// [off[b], off[b + 1]) of box b into out[8 b ..].
+constexpr int kReduceChunk = 4096;
 
-__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part, int n, double* __restrict__ out) {
+__device__ __forceinline__ void reduce8(const double* __restrict__ part, int b, int e, double* __restrict__ out) {
 	__shared__ double sh[8][256];
 	const int t = threadIdx.x;
-	const int chunk = (n + gridDim.x - 1) / gridDim.x;
-	const int b = blockIdx.x * chunk;
-	const int e = min(n, b + chunk);
 	double acc[8];
 	for (int q = 0; q < 8; q++) acc[q] = 0.0;
 	for (int i = b + t; i < e; i += 256) {
@@ -1247,7 +1273,18 @@ __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ part,
 		}
 		__syncthreads();
 	}
-	if (t < 8) out[int64_t(blockIdx.x) * 8 + t] = sh[t][0];
+	if (t < 8) out[t] = sh[t][0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ part, const int2* __restrict__ chunks,
+		double* __restrict__ out) {
+	const int2 r = chunks[blockIdx.x];
+	reduce8(part, r.x, r.y, out + int64_t(blockIdx.x) * 8);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_boxes(const double* __restrict__ chunk_sums, const int32_t* __restrict__ off,
+		double* __restrict__ out) {
+	reduce8(chunk_sums, off[blockIdx.x], off[blockIdx.x + 1], out + int64_t(blockIdx.x) * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -1272,6 +1309,31 @@ __global__ __launch_bounds__(256) void k_drift(uint32_t* __restrict__ x, uint32_
 	if (do_drag) {
 #pragma unroll
 		for (int d = 0; d < 3; d++) V[d][i] = float(double(V[d][i]) * dragc);
+	}
+}
+
+// The drift of every box of a batch in one launch: thread t drifts particle
+// t - off[b] of box b (same arithmetic as k_drift, per-box c1 and drag).
+__global__ __launch_bounds__(256) void k_drift_batch(BoxDesc* __restrict__ boxes, const int32_t* __restrict__ off, int nbox, int total) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= total) return;
+	const int bx = find_box(off, nbox, t);
+	const BoxDesc& B = boxes[bx];
+	const int i = t - off[bx];
+	if (B.rung[i] != B.drift_rung) return;
+	uint32_t* X[3] = {B.x, B.y, B.z};
+	float* V[3] = {B.vx, B.vy, B.vz};
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const double x0 = double(X[d][i]) * (1.0 / 4294967296.0);
+		double x1 = __builtin_fma(double(V[d][i]), B.drift_c1, x0);
+		x1 -= double(x1 >= 1.0);
+		x1 += double(x1 < 0.0);
+		X[d][i] = uint32_t(uint64_t(x1 * 4294967296.0));
+	}
+	if (B.do_drag) {
+#pragma unroll
+		for (int d = 0; d < 3; d++) V[d][i] = float(double(V[d][i]) * B.drag_c);
 	}
 }
 
@@ -1341,6 +1403,7 @@ __global__ void k_fmm(int op, int64_t n, const double* __restrict__ in, double* 
 // ---------------------------------------------------------------------------
 
 thread_local std::string g_err;
+uint64_t g_tree_version = 0;
 
 struct EwaldHost {
 	std::vector<float> rimg, fh, fc;
@@ -1434,22 +1497,6 @@ using namespace ct;
 
 namespace {
 
-int ensure_walk_buffers(ct_box* b) {
-	int rc;
-	for (int q = 0; q < 2; q++) {
-		if ((rc = b->act[q].grow(b->cap_act))) return rc;
-		if ((rc = b->dp[q].grow(b->cap_dp))) return rc;
-		if ((rc = b->ep[q].grow(b->cap_ep))) return rc;
-	}
-	for (int q = 0; q < kNumLists; q++) {
-		if ((rc = b->out[q].grow(b->cap_out[q]))) return rc;
-	}
-	if ((rc = b->seg.grow(b->cap_seg))) return rc;
-	if ((rc = b->head.grow(int64_t(kNumLists) * b->nnodes))) return rc;
-	if ((rc = b->cnt.grow(1))) return rc;
-	return CT_OK;
-}
-
 int upload_box(ct_box* b) {
 	const int64_t n = b->hp.n;
 	int rc;
@@ -1488,92 +1535,9 @@ int download_particles(ct_box* b) {
 	return CT_OK;
 }
 
-// The interaction phases of one kick in arithmetic mode PR: cells (Ewald,
-// M2L, P2L), the L2L sweep, P2P per leaf class, then M2P + L2P + kick.
-template<int PR>
-static void launch_interactions(ct_box* b, hipStream_t st, const ct_kick_params* p) {
-	using T = typename Prec<PR>::T;
-	const void* multi = PR == kF64 ? static_cast<const void*>(b->multi64.p) : static_cast<const void*>(b->multi.p);
-	void* L = PR == kF64 ? static_cast<void*>(b->L64.p) : static_cast<void*>(b->L.p);
-	CellArgs ca;
-	ca.nodes = b->nodes.p;
-	ca.multi = multi;
-	ca.L = L;
-	ca.head = b->head.p;
-	ca.seg = b->seg.p;
-	ca.out_e = b->out[kEwald].p;
-	ca.out_cc = b->out[kCC].p;
-	ca.out_cp = b->out[kCP].p;
-	ca.px = b->x.p;
-	ca.py = b->y.p;
-	ca.pz = b->z.p;
-	ca.nnodes = int32_t(b->nnodes);
-	ca.do_phi = p->do_phi;
-	ca.cnt = b->cnt.p;
-	const int nb = int(std::min<int64_t>(b->nnodes, 1 << 15));
-	hipLaunchKernelGGL((k_cells_ewald<T>), dim3(nb), dim3(64), 0, st, ca);
-	hipLaunchKernelGGL((k_cells_direct<T>), dim3(nb), dim3(64), 0, st, ca);
-	(void) hipEventRecord(b->ev[2], st);
-	for (int d = 1; d <= b->tree.max_depth; d++) {
-		const int lb = b->tree.level_off[d], le = b->tree.level_off[d + 1];
-		if (le > lb)
-			hipLaunchKernelGGL((k_l2l<T>), dim3((le - lb + 63) / 64), dim3(64), 0, st, b->nodes.p, static_cast<T*>(L), lb, le, p->do_phi);
-	}
-	(void) hipEventRecord(b->ev[3], st);
-	PPArgs pa;
-	pa.nodes = b->nodes.p;
-	pa.head = b->head.p;
-	pa.seg = b->seg.p;
-	pa.out_pp = b->out[kPP].p;
-	pa.nnodes = int32_t(b->nnodes);
-	pa.px = b->x.p;
-	pa.py = b->y.p;
-	pa.pz = b->z.p;
-	pa.pp = b->pp.p;
-	pa.h = p->h;
-	pa.cnt = b->cnt.p;
-	for (int c = 0; c < kNumLeafClasses; c++) {
-		const int64_t lo = b->leaf_class_off[c], cnt = b->leaf_class_off[c + 1] - lo;
-		if (cnt == 0) continue;
-		pa.leaf_ids = b->leaf_ids.p + lo;
-		switch (c) {
-#define CT_LAUNCH_PP(C, G, P) \
-	case C: hipLaunchKernelGGL((k_pp<G, P, PR>), dim3(cnt), dim3(64), 0, st, pa); break;
-			CT_LEAF_CLASSES(CT_LAUNCH_PP)
-#undef CT_LAUNCH_PP
-		}
-	}
-	(void) hipEventRecord(b->ev[6], st);
-	LeafArgs la;
-	la.nodes = b->nodes.p;
-	la.multi = multi;
-	la.L = L;
-	la.head = b->head.p;
-	la.seg = b->seg.p;
-	la.out_pc = b->out[kPC].p;
-	la.nnodes = int32_t(b->nnodes);
-	la.px = b->x.p;
-	la.py = b->y.p;
-	la.pz = b->z.p;
-	la.pp = b->pp.p;
-	la.vx = b->vx.p;
-	la.vy = b->vy.p;
-	la.vz = b->vz.p;
-	la.rung = b->rung.p;
-	la.gx = b->gx.p;
-	la.gy = b->gy.p;
-	la.gz = b->gz.p;
-	la.phi = b->phi.p;
-	la.prm = *p;
-	la.cnt = b->cnt.p;
-	la.leaf_ids = b->leaf_ids.p;
-	la.part = b->part.p;
-	if (b->nleaves) hipLaunchKernelGGL((k_kick<PR>), dim3(b->nleaves), dim3(64), 0, st, la);
-}
-
 // FP64 copies of the multipoles and the FP64 local expansions, made on the
 // first FP64 kick after a tree build.
-static int ensure_fp64(ct_box* b) {
+int ensure_fp64(ct_box* b) {
 	const int64_t nn = b->nnodes;
 	int rc;
 	if ((rc = b->L64.grow(std::max<int64_t>(1, nn) * kLStride))) return rc;
@@ -1590,7 +1554,508 @@ static int ensure_fp64(ct_box* b) {
 	return CT_OK;
 }
 
+// The walk arenas, the box table and the work lists of one kick context: a
+// box kicked alone, or a batch of boxes kicked together.
+struct KickCtx {
+	std::vector<ct_box*> boxes;
+	std::vector<uint64_t> versions;  // tree version of each box when prepared
+	bool prepared = false;
+	// box table and index spaces
+	std::vector<BoxDesc> hdesc;
+	DBuf<BoxDesc> desc;
+	DBuf<int32_t> node_off, part_off;
+	int32_t total_nodes = 0, total_parts = 0;
+	int max_depth = 0;
+	DBuf<int32_t> lvl_pref, lvl_lo;  // per level d: [(nbox + 1)] prefix and [nbox] first node
+	std::vector<int32_t> lvl_total;
+	DBuf<int2> pp_leaves;  // (box, leaf) grouped by size class
+	int64_t pp_class_off[kNumLeafClasses + 1] = {0};
+	DBuf<int2> kick_leaves;  // (box, leaf) grouped by box
+	int64_t nkick = 0;
+	DBuf<int2> chunks;  // reduction chunks of kick_leaves, within one box each
+	DBuf<int32_t> chunk_off;
+	int nchunks = 0;
+	DBuf<double> part, chunk_sums, box_sums;
+	// walk arenas
+	DBuf<Act> act[2];
+	DBuf<int32_t> dp[2], ep[2];
+	DBuf<int32_t> out[kNumLists];
+	DBuf<Seg> seg;
+	DBuf<int32_t> head;
+	DBuf<Counters> cnt;
+	int64_t cap_act = 0, cap_dp = 0, cap_ep = 0, cap_out[kNumLists] = {0, 0, 0, 0, 0}, cap_seg = 0;
+	// captured kick (batches); captured on `own` when the caller passes the
+	// default stream, which cannot be captured
+	hipGraphExec_t gexec = nullptr;
+	hipStream_t own = nullptr;
+	int graph_prec = -1;
+	// diagnostics of the last kick
+	hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+	double stats[15] = {0};
+	Counters last_counters;
+	bool have_lists = false;
+	std::vector<double> host_sums;
+
+	int nbox() const { return int(boxes.size()); }
+	void drop_graph() {
+		if (gexec) (void) hipGraphExecDestroy(gexec);
+		gexec = nullptr;
+		graph_prec = -1;
+	}
+	void release() {
+		drop_graph();
+		desc.release();
+		node_off.release();
+		part_off.release();
+		lvl_pref.release();
+		lvl_lo.release();
+		pp_leaves.release();
+		kick_leaves.release();
+		chunks.release();
+		chunk_off.release();
+		part.release();
+		chunk_sums.release();
+		box_sums.release();
+		for (int q = 0; q < 2; q++) {
+			act[q].release();
+			dp[q].release();
+			ep[q].release();
+		}
+		for (int q = 0; q < kNumLists; q++) out[q].release();
+		seg.release();
+		head.release();
+		cnt.release();
+		for (int q = 0; q < 7; q++)
+			if (ev[q]) (void) hipEventDestroy(ev[q]);
+		if (own) (void) hipStreamDestroy(own);
+		own = nullptr;
+	}
+};
+
+int ctx_init(KickCtx* c, const std::vector<ct_box*>& boxes) {
+	c->boxes = boxes;
+	c->versions.assign(boxes.size(), ~uint64_t(0));
+	for (int q = 0; q < 7; q++) CT_HIP(hipEventCreate(&c->ev[q]));
+	CT_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+	return CT_OK;
+}
+
+// (Re)builds the index spaces and work lists after a box's tree changed.
+int ctx_prepare(KickCtx* c) {
+	const int nb = c->nbox();
+	bool stale = !c->prepared;
+	for (int b = 0; b < nb; b++) {
+		if (!c->boxes[b]->have_tree) return fail(CT_E_STATE, "kick: a box has no tree (call ct_tree_build)");
+		stale = stale || c->versions[b] != c->boxes[b]->tree_version;
+	}
+	if (!stale) return CT_OK;
+	c->drop_graph();
+	int rc;
+	std::vector<int32_t> noff(nb + 1, 0), poff(nb + 1, 0);
+	int maxd = 0;
+	for (int b = 0; b < nb; b++) {
+		noff[b + 1] = noff[b] + int32_t(c->boxes[b]->nnodes);
+		poff[b + 1] = poff[b] + int32_t(c->boxes[b]->hp.n);
+		maxd = std::max(maxd, c->boxes[b]->tree.max_depth);
+	}
+	c->total_nodes = noff[nb];
+	c->total_parts = poff[nb];
+	c->max_depth = maxd;
+	// levels 1..maxd of every box
+	std::vector<int32_t> pref((maxd + 1) * (nb + 1), 0), lo((maxd + 1) * std::max(nb, 1), 0);
+	c->lvl_total.assign(maxd + 1, 0);
+	for (int d = 0; d <= maxd; d++) {
+		for (int b = 0; b < nb; b++) {
+			const HostTree& T = c->boxes[b]->tree;
+			int32_t l0 = 0, l1 = 0;
+			if (d <= T.max_depth) {
+				l0 = T.level_off[d];
+				l1 = T.level_off[d + 1];
+			}
+			lo[d * nb + b] = l0;
+			pref[d * (nb + 1) + b + 1] = pref[d * (nb + 1) + b] + (l1 - l0);
+		}
+		c->lvl_total[d] = pref[d * (nb + 1) + nb];
+	}
+	// leaf lists
+	std::vector<int2> ppl, kl;
+	for (int cl = 0; cl < kNumLeafClasses; cl++) {
+		c->pp_class_off[cl] = int64_t(ppl.size());
+		for (int b = 0; b < nb; b++)
+			for (int32_t l : c->boxes[b]->leaf_class[cl]) ppl.push_back(make_int2(b, l));
+	}
+	c->pp_class_off[kNumLeafClasses] = int64_t(ppl.size());
+	std::vector<int2> ch;
+	std::vector<int32_t> choff(nb + 1, 0);
+	for (int b = 0; b < nb; b++) {
+		const int64_t k0 = int64_t(kl.size());
+		for (int cl = 0; cl < kNumLeafClasses; cl++)
+			for (int32_t l : c->boxes[b]->leaf_class[cl]) kl.push_back(make_int2(b, l));
+		const int64_t k1 = int64_t(kl.size());
+		for (int64_t q = k0; q < k1; q += kReduceChunk) ch.push_back(make_int2(int(q), int(std::min<int64_t>(k1, q + kReduceChunk))));
+		if (k1 == k0) ch.push_back(make_int2(int(k0), int(k0)));
+		choff[b + 1] = int32_t(ch.size());
+	}
+	c->nkick = int64_t(kl.size());
+	c->nchunks = int(ch.size());
+	if ((rc = c->desc.grow(std::max(nb, 1))) || (rc = c->node_off.grow(nb + 1)) || (rc = c->part_off.grow(nb + 1)) ||
+			(rc = c->lvl_pref.grow(int64_t(pref.size()))) || (rc = c->lvl_lo.grow(int64_t(lo.size()))) ||
+			(rc = c->pp_leaves.grow(std::max<int64_t>(1, int64_t(ppl.size())))) || (rc = c->kick_leaves.grow(std::max<int64_t>(1, c->nkick))) ||
+			(rc = c->chunks.grow(std::max(1, c->nchunks))) || (rc = c->chunk_off.grow(nb + 1)) ||
+			(rc = c->part.grow(std::max<int64_t>(1, c->nkick) * 8)) || (rc = c->chunk_sums.grow(int64_t(std::max(1, c->nchunks)) * 8)) ||
+			(rc = c->box_sums.grow(int64_t(std::max(1, nb)) * 8)) || (rc = c->head.grow(std::max<int64_t>(1, int64_t(kNumLists) * c->total_nodes))) ||
+			(rc = c->cnt.grow(1)))
+		return rc;
+	CT_HIP(hipMemcpy(c->node_off.p, noff.data(), noff.size() * 4, hipMemcpyHostToDevice));
+	CT_HIP(hipMemcpy(c->part_off.p, poff.data(), poff.size() * 4, hipMemcpyHostToDevice));
+	CT_HIP(hipMemcpy(c->lvl_pref.p, pref.data(), pref.size() * 4, hipMemcpyHostToDevice));
+	CT_HIP(hipMemcpy(c->lvl_lo.p, lo.data(), lo.size() * 4, hipMemcpyHostToDevice));
+	if (!ppl.empty()) CT_HIP(hipMemcpy(c->pp_leaves.p, ppl.data(), ppl.size() * sizeof(int2), hipMemcpyHostToDevice));
+	if (!kl.empty()) CT_HIP(hipMemcpy(c->kick_leaves.p, kl.data(), kl.size() * sizeof(int2), hipMemcpyHostToDevice));
+	if (!ch.empty()) CT_HIP(hipMemcpy(c->chunks.p, ch.data(), ch.size() * sizeof(int2), hipMemcpyHostToDevice));
+	CT_HIP(hipMemcpy(c->chunk_off.p, choff.data(), choff.size() * 4, hipMemcpyHostToDevice));
+	// box table: everything but the per-kick fields
+	c->hdesc.assign(nb, BoxDesc());
+	for (int b = 0; b < nb; b++) {
+		ct_box* x = c->boxes[b];
+		BoxDesc& d = c->hdesc[b];
+		std::memset(&d, 0, sizeof(d));
+		d.nodes = x->nodes.p;
+		d.x = x->x.p;
+		d.y = x->y.p;
+		d.z = x->z.p;
+		d.vx = x->vx.p;
+		d.vy = x->vy.p;
+		d.vz = x->vz.p;
+		d.rung = x->rung.p;
+		d.gx = x->gx.p;
+		d.gy = x->gy.p;
+		d.gz = x->gz.p;
+		d.phi = x->phi.p;
+		d.head = c->head.p + int64_t(kNumLists) * noff[b];
+		d.pp = x->pp.p;
+		d.nnodes = int32_t(x->nnodes);
+		d.nparts = int32_t(x->hp.n);
+		c->versions[b] = x->tree_version;
+	}
+	// initial list capacities from the node count; grown on overflow
+	const int64_t nn = std::max<int64_t>(c->total_nodes, 1);
+	c->cap_act = std::max<int64_t>({c->cap_act, 4 * nn, 1024});
+	c->cap_dp = std::max<int64_t>({c->cap_dp, 64 * nn, 1 << 16});
+	c->cap_ep = std::max<int64_t>({c->cap_ep, 16 * nn, 1 << 14});
+	c->cap_out[kEwald] = std::max<int64_t>({c->cap_out[kEwald], 8 * nn, 1 << 14});
+	c->cap_out[kCC] = std::max<int64_t>({c->cap_out[kCC], 64 * nn, 1 << 16});
+	c->cap_out[kCP] = std::max<int64_t>({c->cap_out[kCP], 4 * nn, 1 << 12});
+	c->cap_out[kPC] = std::max<int64_t>({c->cap_out[kPC], 4 * nn, 1 << 12});
+	c->cap_out[kPP] = std::max<int64_t>({c->cap_out[kPP], 32 * nn, 1 << 16});
+	c->cap_seg = std::max<int64_t>({c->cap_seg, 8 * nn, 1 << 14});
+	c->prepared = true;
+	c->have_lists = false;
+	return CT_OK;
+}
+
+int ctx_grow_arenas(KickCtx* c) {
+	int rc;
+	for (int q = 0; q < 2; q++) {
+		if ((rc = c->act[q].grow(c->cap_act))) return rc;
+		if ((rc = c->dp[q].grow(c->cap_dp))) return rc;
+		if ((rc = c->ep[q].grow(c->cap_ep))) return rc;
+	}
+	for (int q = 0; q < kNumLists; q++) {
+		if ((rc = c->out[q].grow(c->cap_out[q]))) return rc;
+	}
+	if ((rc = c->seg.grow(c->cap_seg))) return rc;
+	return CT_OK;
+}
+
+// Per-kick fields of the box table: parameters and the arithmetic's buffers.
+int ctx_set_params(KickCtx* c, const ct_kick_params* p, int precision) {
+	int rc;
+	for (int b = 0; b < c->nbox(); b++) {
+		ct_box* x = c->boxes[b];
+		if (p[b].min_rung < 0 || p[b].min_rung >= 16 || !(p[b].theta > 0.0f) || !(p[b].h > 0.0f)) return fail(CT_E_ARG, "kick: bad params");
+		if (precision == CT_PREC_F64 && (rc = ensure_fp64(x))) return rc;
+		BoxDesc& d = c->hdesc[b];
+		d.multi = precision == CT_PREC_F64 ? static_cast<const void*>(x->multi64.p) : static_cast<const void*>(x->multi.p);
+		d.L = precision == CT_PREC_F64 ? static_cast<void*>(x->L64.p) : static_cast<void*>(x->L.p);
+		d.theta_inv = 1.0f / p[b].theta;
+		d.h = p[b].h;
+		d.prm = p[b];
+	}
+	return CT_OK;
+}
+
+WalkArgs walk_args(KickCtx* c) {
+	WalkArgs A;
+	A.boxes = c->desc.p;
+	for (int q = 0; q < 2; q++) {
+		A.act[q] = c->act[q].p;
+		A.dp[q] = c->dp[q].p;
+		A.ep[q] = c->ep[q].p;
+	}
+	A.cap_act = c->cap_act;
+	A.cap_dp = c->cap_dp;
+	A.cap_ep = c->cap_ep;
+	for (int q = 0; q < kNumLists; q++) {
+		A.out[q] = c->out[q].p;
+		A.cap_out[q] = c->cap_out[q];
+	}
+	A.seg = c->seg.p;
+	A.cap_seg = c->cap_seg;
+	A.cnt = c->cnt.p;
+	return A;
+}
+
+constexpr int kWalkGrid = 1024;  // 4096 waves, all resident at the walk's register use
+
+int walk_iters(KickCtx* c) {
+	return c->max_depth + 3;
+}
+
+// The level launches of one walk (no host synchronisation).
+void launch_walk(KickCtx* c, hipStream_t st) {
+	InitArgs ia;
+	ia.head = c->head.p;
+	ia.nhead = int64_t(kNumLists) * c->total_nodes;
+	ia.cnt = c->cnt.p;
+	ia.act0 = c->act[0].p;
+	ia.dp0 = c->dp[0].p;
+	ia.ep0 = c->ep[0].p;
+	ia.nbox = c->nbox();
+	hipLaunchKernelGGL(k_walk_init, dim3(256), dim3(256), 0, st, ia);
+	hipLaunchKernelGGL(k_walk_seed_counts, dim3(1), dim3(64), 0, st, c->cnt.p, c->nbox());
+	const WalkArgs A = walk_args(c);
+	const int iters = walk_iters(c);
+	for (int k = 0; k < iters; k++) {
+		hipLaunchKernelGGL(k_walk, dim3(kWalkGrid), dim3(256), 0, st, A, k);
+	}
+}
+
+// Walk with overflow recovery: arenas that overflowed are grown from the
+// counters and the walk is run again.
+int run_walk(KickCtx* c, hipStream_t st) {
+	if (walk_iters(c) > kMaxIter) return fail(CT_E_TREE, "tree too deep for the walk");
+	for (int attempt = 0; attempt < 12; attempt++) {
+		int rc = ctx_grow_arenas(c);
+		if (rc) return rc;
+		launch_walk(c, st);
+		Counters C;
+		CT_HIP(hipMemcpyAsync(&C, c->cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+		CT_HIP(hipStreamSynchronize(st));
+		const int iters = walk_iters(c);
+		if (C.walk_error) return fail(CT_E_WALK, "walk: a leaf failed the Ewald acceptance test");
+		if (C.act[iters] != 0 && !C.overflow) return fail(CT_E_WALK, "walk did not terminate within the tree depth");
+		if (!C.overflow) return CT_OK;
+		c->drop_graph();
+		auto bump = [](int64_t& cap, int64_t need) { cap = std::max<int64_t>(2 * cap, need + need / 4 + 1024); };
+		int maxa = 0, maxd = 0, maxe = 0;
+		for (int k = 0; k <= iters; k++) {
+			maxa = std::max(maxa, C.act[k]);
+			maxd = std::max(maxd, C.dp[k]);
+			maxe = std::max(maxe, C.ep[k]);
+		}
+		if (C.overflow & (1 << 5)) bump(c->cap_act, maxa);
+		if (C.overflow & (1 << 6)) bump(c->cap_dp, maxd);
+		if (C.overflow & (1 << 7)) bump(c->cap_ep, maxe);
+		for (int q = 0; q < kNumLists; q++)
+			if (C.overflow & (1 << q)) bump(c->cap_out[q], C.out[q]);
+		if (C.overflow & (1 << 8)) bump(c->cap_seg, C.seg);
+	}
+	return fail(CT_E_ARENA, "walk: list buffers kept overflowing");
+}
+
+// The interaction phases of one kick in arithmetic mode PR: cells (Ewald,
+// M2L, P2L), the L2L sweep, P2P per leaf class, M2P + L2P + kick, and the
+// per-box reduction of the kick_return sums.
+template<int PR>
+void launch_interactions(KickCtx* c, hipStream_t st, bool events) {
+	using T = typename Prec<PR>::T;
+	const int nb = c->nbox();
+	CellArgs ca;
+	ca.boxes = c->desc.p;
+	ca.node_off = c->node_off.p;
+	ca.nbox = nb;
+	ca.total = c->total_nodes;
+	ca.seg = c->seg.p;
+	ca.out_e = c->out[kEwald].p;
+	ca.out_cc = c->out[kCC].p;
+	ca.out_cp = c->out[kCP].p;
+	ca.cnt = c->cnt.p;
+	const int ng = std::max(1, std::min(c->total_nodes, 1 << 15));
+	hipLaunchKernelGGL((k_cells_ewald<T>), dim3(ng), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL((k_cells_direct<T>), dim3(ng), dim3(64), 0, st, ca);
+	if (events) (void) hipEventRecord(c->ev[2], st);
+	for (int d = 1; d <= c->max_depth; d++) {
+		const int tot = c->lvl_total[d];
+		if (tot == 0) continue;
+		L2LArgs la;
+		la.boxes = c->desc.p;
+		la.pref = c->lvl_pref.p + d * (nb + 1);
+		la.lo = c->lvl_lo.p + d * nb;
+		la.nbox = nb;
+		la.total = tot;
+		hipLaunchKernelGGL((k_l2l<T>), dim3((tot + 63) / 64), dim3(64), 0, st, la);
+	}
+	if (events) (void) hipEventRecord(c->ev[3], st);
+	PPArgs pa;
+	pa.boxes = c->desc.p;
+	pa.seg = c->seg.p;
+	pa.out_pp = c->out[kPP].p;
+	pa.cnt = c->cnt.p;
+	for (int cl = 0; cl < kNumLeafClasses; cl++) {
+		const int64_t lo = c->pp_class_off[cl], cnt = c->pp_class_off[cl + 1] - lo;
+		if (cnt == 0) continue;
+		pa.leaves = c->pp_leaves.p + lo;
+		switch (cl) {
+#define CT_LAUNCH_PP(C, G, P) \
+	case C: hipLaunchKernelGGL((k_pp<G, P, PR>), dim3(cnt), dim3(64), 0, st, pa); break;
+			CT_LEAF_CLASSES(CT_LAUNCH_PP)
+#undef CT_LAUNCH_PP
+		}
+	}
+	if (events) (void) hipEventRecord(c->ev[6], st);
+	LeafArgs lk;
+	lk.boxes = c->desc.p;
+	lk.leaves = c->kick_leaves.p;
+	lk.seg = c->seg.p;
+	lk.out_pc = c->out[kPC].p;
+	lk.part = c->part.p;
+	lk.cnt = c->cnt.p;
+	if (c->nkick) hipLaunchKernelGGL((k_kick<PR>), dim3(c->nkick), dim3(64), 0, st, lk);
+	if (events) (void) hipEventRecord(c->ev[4], st);
+	hipLaunchKernelGGL(k_reduce_chunks, dim3(c->nchunks), dim3(256), 0, st, c->part.p, c->chunks.p, c->chunk_sums.p);
+	hipLaunchKernelGGL(k_reduce_boxes, dim3(nb), dim3(256), 0, st, c->chunk_sums.p, c->chunk_off.p, c->box_sums.p);
+}
+
+void launch_interactions_prec(KickCtx* c, hipStream_t st, int precision, bool events) {
+	switch (precision) {
+	case CT_PREC_F32U:
+		launch_interactions<kF32U>(c, st, events);
+		break;
+	case CT_PREC_F64:
+		launch_interactions<kF64>(c, st, events);
+		break;
+	default:
+		launch_interactions<kF32C>(c, st, events);
+		break;
+	}
+}
+
+// Results of the last kick: returns, counters and phase times.
+int ctx_finish(KickCtx* c, hipStream_t st, ct_kick_return* rets, bool events) {
+	const int nb = c->nbox();
+	c->host_sums.assign(int64_t(std::max(nb, 1)) * 8, 0.0);
+	Counters C;
+	CT_HIP(hipMemcpyAsync(c->host_sums.data(), c->box_sums.p, int64_t(nb) * 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+	CT_HIP(hipMemcpyAsync(&C, c->cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
+	CT_HIP(hipEventRecord(c->ev[5], st));
+	CT_HIP(hipStreamSynchronize(st));
+	CT_HIP(hipGetLastError());
+	if (C.walk_error) return fail(CT_E_WALK, "walk: a leaf failed the Ewald acceptance test");
+	for (ct_box* x : c->boxes) x->host_stale = true;
+	c->have_lists = true;
+	c->last_counters = C;
+	for (int b = 0; b < nb && rets; b++) {
+		const double* r = c->host_sums.data() + int64_t(b) * 8;
+		rets[b].max_rung = int(r[0]);
+		rets[b].pot = r[1];
+		rets[b].kin = r[2];
+		rets[b].dkin = r[3];
+		rets[b].xmom = r[4];
+		rets[b].ymom = r[5];
+		rets[b].zmom = r[6];
+		rets[b].nmom = r[7];
+	}
+	float ms = 0.0f;
+	if (events) {
+		for (int q = 0; q < 4; q++) {
+			CT_HIP(hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]));
+			c->stats[q] = ms;
+		}
+		CT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[6]));
+		c->stats[12] = ms;
+		CT_HIP(hipEventElapsedTime(&ms, c->ev[6], c->ev[4]));
+		c->stats[13] = ms;
+	} else {
+		for (int q = 0; q < 4; q++) c->stats[q] = 0.0;
+		c->stats[12] = c->stats[13] = 0.0;
+	}
+	CT_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[5]));
+	c->stats[4] = ms;
+	c->stats[5] = double(C.stat[0]);
+	c->stats[6] = double(C.stat[1]);
+	c->stats[7] = double(C.stat[2]);
+	c->stats[8] = double(C.stat[3]);
+	c->stats[9] = double(C.stat[4]);
+	c->stats[10] = double(c->total_nodes);
+	c->stats[11] = double(C.stat[6]);
+	c->stats[14] = double(C.stat[7]);
+	return CT_OK;
+}
+
+// One kick of every box of the context.  Eager: the walk, a check of its
+// counters (arenas grown and the walk repeated on overflow), then the
+// interactions.  Graph: the whole kick replayed from a captured graph; if the
+// walk overflowed in the replay, the kick is repeated eagerly and the graph
+// captured again on the next call.
+int ctx_kick(KickCtx* c, const ct_kick_params* p, int precision, ct_kick_return* rets, bool graph, hipStream_t st) {
+	int rc;
+	if (precision != CT_PREC_F32C && precision != CT_PREC_F32U && precision != CT_PREC_F64) return fail(CT_E_ARG, "kick: bad precision");
+	if ((rc = ctx_prepare(c))) return rc;
+	if ((rc = ctx_set_params(c, p, precision))) return rc;
+	if (walk_iters(c) > kMaxIter) return fail(CT_E_TREE, "tree too deep for the walk");
+	if (graph && st == nullptr) st = c->own;
+	CT_HIP(hipMemcpyAsync(c->desc.p, c->hdesc.data(), c->hdesc.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, st));
+	if (graph && c->gexec && c->graph_prec == precision) {
+		CT_HIP(hipEventRecord(c->ev[0], st));
+		CT_HIP(hipGraphLaunch(c->gexec, st));
+		if ((rc = ctx_finish(c, st, rets, false))) return rc;
+		if (!c->last_counters.overflow) return CT_OK;
+		c->drop_graph();  // overflowed: fall through to an eager kick
+	}
+	CT_HIP(hipEventRecord(c->ev[0], st));
+	if ((rc = run_walk(c, st))) return rc;
+	CT_HIP(hipEventRecord(c->ev[1], st));
+	launch_interactions_prec(c, st, precision, true);
+	if ((rc = ctx_finish(c, st, rets, true))) return rc;
+	if (graph && !c->gexec) {
+		// capture walk + interactions for the next calls (arenas now fit)
+		hipGraph_t g = nullptr;
+		CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+		launch_walk(c, st);
+		launch_interactions_prec(c, st, precision, false);
+		hipError_t e = hipStreamEndCapture(st, &g);
+		if (e != hipSuccess) return fail(CT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+		e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+		(void) hipGraphDestroy(g);
+		if (e != hipSuccess) {
+			c->gexec = nullptr;
+			return fail(CT_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+		}
+		c->graph_prec = precision;
+	}
+	return CT_OK;
+}
+
+KickCtx* box_ctx(ct_box* b) {
+	if (!b->self_ctx) {
+		KickCtx* c = new (std::nothrow) KickCtx();
+		if (!c) return nullptr;
+		if (ctx_init(c, {b}) != CT_OK) {
+			c->release();
+			delete c;
+			return nullptr;
+		}
+		b->self_ctx = c;
+	}
+	return static_cast<KickCtx*>(b->self_ctx);
+}
+
 }  // namespace
+
+struct ct_batch {
+	KickCtx ctx;
+};
 
 int ct::check_init() {
 	if (g_dev.device < 0) return fail(CT_E_STATE, "ct_init has not been called");
@@ -1638,12 +2103,6 @@ int ct_box_create(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_
 		P.rung[i] = rung ? rung[i] : 0;
 		P.id[i] = int32_t(i);
 	}
-	for (int q = 0; q < 7; q++) {
-		if (hipEventCreate(&b->ev[q]) != hipSuccess) {
-			delete b;
-			return fail(CT_E_HIP, "hipEventCreate failed");
-		}
-	}
 	if ((rc = upload_box(b))) {
 		ct_box_destroy(b);
 		return rc;
@@ -1655,6 +2114,12 @@ int ct_box_create(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_
 int ct_box_destroy(ct_box_t b) {
 	if (!b) return CT_OK;
 	(void) hipDeviceSynchronize();
+	if (b->self_ctx) {
+		KickCtx* c = static_cast<KickCtx*>(b->self_ctx);
+		c->release();
+		delete c;
+		b->self_ctx = nullptr;
+	}
 	b->x.release();
 	b->y.release();
 	b->z.release();
@@ -1672,20 +2137,6 @@ int ct_box_destroy(ct_box_t b) {
 	b->multi64.release();
 	b->L64.release();
 	b->pp.release();
-	b->leaf_ids.release();
-	for (int q = 0; q < 2; q++) {
-		b->act[q].release();
-		b->dp[q].release();
-		b->ep[q].release();
-	}
-	for (int q = 0; q < kNumLists; q++) b->out[q].release();
-	b->seg.release();
-	b->head.release();
-	b->cnt.release();
-	b->part.release();
-	b->red.release();
-	for (int q = 0; q < 7; q++)
-		if (b->ev[q]) (void) hipEventDestroy(b->ev[q]);
 	delete b;
 	return CT_OK;
 }
@@ -1702,17 +2153,12 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 	std::string err;
 	HostTree T;
 	if ((rc = build_tree(b->hp, prm, T, &err))) return fail(rc, err);
-	b->tree = std::move(T);
-	b->theta = theta;
-	b->bucket = bucket;
-	b->hsoft = hsoft;
-	const int64_t nn = int64_t(b->tree.nodes.size());
-	b->nnodes = nn;
+	const int64_t nn = int64_t(T.nodes.size());
 	std::vector<DNode> dn(nn);
 	std::vector<float> mp(nn * kMultiStride, 0.0f);
 	std::vector<int32_t> cls[kNumLeafClasses];
 	for (int64_t i = 0; i < nn; i++) {
-		const HostNode& h = b->tree.nodes[i];
+		const HostNode& h = T.nodes[i];
 		DNode& d = dn[i];
 		for (int q = 0; q < 3; q++) {
 			d.pos[q] = h.pos[q];
@@ -1733,105 +2179,23 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 			cls[leaf_class(h.pend - h.pbegin)].push_back(int32_t(i));
 		}
 	}
-	// non-empty leaves grouped by size class, one launch per class
-	std::vector<int32_t> leaves;
-	b->leaf_class_off[0] = 0;
-	for (int c = 0; c < kNumLeafClasses; c++) {
-		leaves.insert(leaves.end(), cls[c].begin(), cls[c].end());
-		b->leaf_class_off[c + 1] = int64_t(leaves.size());
-	}
-	b->nleaves = int64_t(leaves.size());
+	(void) hipDeviceSynchronize();
 	if ((rc = b->nodes.grow(nn)) || (rc = b->multi.grow(nn * kMultiStride)) || (rc = b->L.grow(nn * kLStride)) ||
-			(rc = b->leaf_ids.grow(std::max<int64_t>(1, b->nleaves))) || (rc = b->part.grow(std::max<int64_t>(1, b->nleaves) * 8)) ||
-			(rc = b->red.grow(8 * (kReduceBlocks + 1))) || (rc = b->pp.grow(std::max<int64_t>(1, b->hp.n))))
+			(rc = b->pp.grow(std::max<int64_t>(1, b->hp.n))))
 		return rc;
 	CT_HIP(hipMemcpy(b->nodes.p, dn.data(), nn * sizeof(DNode), hipMemcpyHostToDevice));
 	CT_HIP(hipMemcpy(b->multi.p, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice));
-	if (b->nleaves) CT_HIP(hipMemcpy(b->leaf_ids.p, leaves.data(), leaves.size() * sizeof(int32_t), hipMemcpyHostToDevice));
 	if ((rc = upload_box(b))) return rc;
+	b->tree = std::move(T);
+	b->theta = theta;
+	b->bucket = bucket;
+	b->hsoft = hsoft;
+	b->nnodes = nn;
+	for (int c = 0; c < kNumLeafClasses; c++) b->leaf_class[c] = std::move(cls[c]);
 	b->have_tree = true;
-	b->have_lists = false;
 	b->multi64_ready = false;
-	// initial list capacities; grown from the counters on overflow
-	if (b->cap_act == 0) {
-		b->cap_act = std::max<int64_t>(4 * nn, 1024);
-		b->cap_dp = std::max<int64_t>(64 * nn, 1 << 16);
-		b->cap_ep = std::max<int64_t>(16 * nn, 1 << 14);
-		b->cap_out[kEwald] = std::max<int64_t>(8 * nn, 1 << 14);
-		b->cap_out[kCC] = std::max<int64_t>(64 * nn, 1 << 16);
-		b->cap_out[kCP] = std::max<int64_t>(4 * nn, 1 << 12);
-		b->cap_out[kPC] = std::max<int64_t>(4 * nn, 1 << 12);
-		b->cap_out[kPP] = std::max<int64_t>(32 * nn, 1 << 16);
-		b->cap_seg = std::max<int64_t>(8 * nn, 1 << 14);
-	}
+	b->tree_version = ++g_tree_version;
 	return CT_OK;
-}
-
-static int run_walk(ct_box* b, hipStream_t st, const ct_kick_params* p) {
-	const int iters = b->tree.max_depth + 3;
-	if (iters > kMaxIter) return fail(CT_E_TREE, "tree too deep for the walk");
-	for (int attempt = 0; attempt < 12; attempt++) {
-		int rc = ensure_walk_buffers(b);
-		if (rc) return rc;
-		InitArgs ia;
-		ia.head = b->head.p;
-		ia.nhead = int64_t(kNumLists) * b->nnodes;
-		ia.cnt = b->cnt.p;
-		ia.act0 = b->act[0].p;
-		ia.dp0 = b->dp[0].p;
-		ia.ep0 = b->ep[0].p;
-		ia.root = 0;
-		hipLaunchKernelGGL(k_walk_init, dim3(256), dim3(256), 0, st, ia);
-		hipLaunchKernelGGL(k_walk_seed_counts, dim3(1), dim3(64), 0, st, b->cnt.p);
-		WalkArgs A;
-		A.nodes = b->nodes.p;
-		A.nnodes = int32_t(b->nnodes);
-		for (int q = 0; q < 2; q++) {
-			A.act[q] = b->act[q].p;
-			A.dp[q] = b->dp[q].p;
-			A.ep[q] = b->ep[q].p;
-		}
-		A.cap_act = b->cap_act;
-		A.cap_dp = b->cap_dp;
-		A.cap_ep = b->cap_ep;
-		for (int q = 0; q < kNumLists; q++) {
-			A.out[q] = b->out[q].p;
-			A.cap_out[q] = b->cap_out[q];
-		}
-		A.seg = b->seg.p;
-		A.cap_seg = b->cap_seg;
-		A.head = b->head.p;
-		A.cnt = b->cnt.p;
-		A.theta_inv = 1.0f / p->theta;
-		A.h = p->h;
-		const int grid = 1024;  // 4096 waves, all resident at the walk's register use
-		for (int k = 0; k < iters; k++) {
-			hipLaunchKernelGGL(k_walk, dim3(grid), dim3(256), 0, st, A, k);
-		}
-		Counters C;
-		CT_HIP(hipMemcpyAsync(&C, b->cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
-		CT_HIP(hipStreamSynchronize(st));
-		if (C.walk_error) return fail(CT_E_WALK, "walk: a leaf failed the Ewald acceptance test");
-		if (C.act[iters] != 0 && !C.overflow) return fail(CT_E_WALK, "walk did not terminate within the tree depth");
-		if (!C.overflow) {
-			b->last_counters = C;
-			return CT_OK;
-		}
-		auto bump = [](int64_t& cap, int64_t need) { cap = std::max<int64_t>(2 * cap, need + need / 4 + 1024); };
-		int maxa = 0, maxd = 0, maxe = 0;
-		for (int k = 0; k <= iters; k++) {
-			maxa = std::max(maxa, C.act[k]);
-			maxd = std::max(maxd, C.dp[k]);
-			maxe = std::max(maxe, C.ep[k]);
-		}
-		if (C.overflow & (1 << 5)) bump(b->cap_act, maxa);
-		if (C.overflow & (1 << 6)) bump(b->cap_dp, maxd);
-		if (C.overflow & (1 << 7)) bump(b->cap_ep, maxe);
-		for (int q = 0; q < kNumLists; q++)
-			if (C.overflow & (1 << q)) bump(b->cap_out[q], C.out[q]);
-		if (C.overflow & (1 << 8)) bump(b->cap_seg, C.seg);
-	}
-	return fail(CT_E_ARENA, "walk: list buffers kept overflowing");
 }
 
 int ct_kick(ct_box_t b, const ct_kick_params* p, ct_kick_return* ret, void* stream) {
@@ -1842,67 +2206,76 @@ int ct_kick_prec(ct_box_t b, const ct_kick_params* p, int precision, ct_kick_ret
 	int rc = check_init();
 	if (rc) return rc;
 	if (!b || !p) return fail(CT_E_ARG, "ct_kick: bad arguments");
-	if (precision != CT_PREC_F32C && precision != CT_PREC_F32U && precision != CT_PREC_F64) return fail(CT_E_ARG, "ct_kick: bad precision");
 	if (!b->have_tree) return fail(CT_E_STATE, "ct_kick: no tree (call ct_tree_build)");
-	if (p->min_rung < 0 || p->min_rung >= 16 || !(p->theta > 0.0f) || !(p->h > 0.0f)) return fail(CT_E_ARG, "ct_kick: bad params");
-	if (precision == CT_PREC_F64 && (rc = ensure_fp64(b))) return rc;
+	KickCtx* c = box_ctx(b);
+	if (!c) return fail(CT_E_OOM, "ct_kick: host allocation failed");
+	return ctx_kick(c, p, precision, ret, false, reinterpret_cast<hipStream_t>(stream));
+}
+
+int ct_batch_create(const ct_box_t* boxes, int nbox, ct_batch_t* out) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!out || nbox < 1 || !boxes) return fail(CT_E_ARG, "ct_batch_create: bad arguments");
+	for (int i = 0; i < nbox; i++) {
+		if (!boxes[i]) return fail(CT_E_ARG, "ct_batch_create: null box");
+		for (int j = 0; j < i; j++)
+			if (boxes[j] == boxes[i]) return fail(CT_E_ARG, "ct_batch_create: a box appears twice");
+	}
+	ct_batch* bt = new (std::nothrow) ct_batch();
+	if (!bt) return fail(CT_E_OOM, "ct_batch_create: host allocation failed");
+	if ((rc = ctx_init(&bt->ctx, std::vector<ct_box*>(boxes, boxes + nbox)))) {
+		bt->ctx.release();
+		delete bt;
+		return rc;
+	}
+	*out = bt;
+	return CT_OK;
+}
+
+int ct_batch_destroy(ct_batch_t bt) {
+	if (!bt) return CT_OK;
+	(void) hipDeviceSynchronize();
+	bt->ctx.release();
+	delete bt;
+	return CT_OK;
+}
+
+int ct_batch_kick(ct_batch_t bt, const ct_kick_params* params, int precision, ct_kick_return* rets, int use_graph, void* stream) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!bt || !params) return fail(CT_E_ARG, "ct_batch_kick: bad arguments");
+	return ctx_kick(&bt->ctx, params, precision, rets, use_graph != 0, reinterpret_cast<hipStream_t>(stream));
+}
+
+int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, const double* drag, void* stream) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!bt || !a || !dt) return fail(CT_E_ARG, "ct_batch_drift: bad arguments");
+	KickCtx* c = &bt->ctx;
+	if ((rc = ctx_prepare(c))) return rc;
+	for (int b = 0; b < c->nbox(); b++) {
+		if (!(a[b] > 0.0f)) return fail(CT_E_ARG, "ct_batch_drift: bad scale factor");
+		BoxDesc& d = c->hdesc[b];
+		const double dg = drag ? drag[b] : 0.0;
+		d.drift_c1 = double(dt[b]) / double(a[b]);
+		d.drag_c = 1.0 / (1.0 + dg * double(dt[b]));
+		d.do_drag = dg != 0.0;
+		d.drift_rung = rung;
+	}
 	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-	CT_HIP(hipEventRecord(b->ev[0], st));
-	if ((rc = run_walk(b, st, p))) return rc;
-	CT_HIP(hipEventRecord(b->ev[1], st));
-	switch (precision) {
-	case CT_PREC_F32U:
-		launch_interactions<kF32U>(b, st, p);
-		break;
-	case CT_PREC_F64:
-		launch_interactions<kF64>(b, st, p);
-		break;
-	default:
-		launch_interactions<kF32C>(b, st, p);
-		break;
-	}
-	CT_HIP(hipEventRecord(b->ev[4], st));
-	hipLaunchKernelGGL(k_reduce, dim3(kReduceBlocks), dim3(256), 0, st, b->part.p, int(b->nleaves), b->red.p + 8);
-	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, b->red.p + 8, kReduceBlocks, b->red.p);
-	double red[8];
-	CT_HIP(hipMemcpyAsync(red, b->red.p, sizeof(red), hipMemcpyDeviceToHost, st));
-	CT_HIP(hipEventRecord(b->ev[5], st));
-	Counters C;
-	CT_HIP(hipMemcpyAsync(&C, b->cnt.p, sizeof(Counters), hipMemcpyDeviceToHost, st));
-	CT_HIP(hipStreamSynchronize(st));
+	CT_HIP(hipMemcpyAsync(c->desc.p, c->hdesc.data(), c->hdesc.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, st));
+	if (c->total_parts > 0)
+		hipLaunchKernelGGL(k_drift_batch, dim3((c->total_parts + 255) / 256), dim3(256), 0, st, c->desc.p, c->part_off.p, c->nbox(),
+				c->total_parts);
 	CT_HIP(hipGetLastError());
-	b->host_stale = true;
-	b->have_lists = true;
-	b->last_counters = C;
-	if (ret) {
-		ret->max_rung = int(red[0]);
-		ret->pot = red[1];
-		ret->kin = red[2];
-		ret->dkin = red[3];
-		ret->xmom = red[4];
-		ret->ymom = red[5];
-		ret->zmom = red[6];
-		ret->nmom = red[7];
-	}
-	float ms;
-	for (int q = 0; q < 4; q++) {
-		CT_HIP(hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]));
-		b->stats[q] = ms;
-	}
-	CT_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[4]));
-	b->stats[4] = ms;
-	b->stats[5] = double(C.stat[0]);
-	b->stats[6] = double(C.stat[1]);
-	b->stats[7] = double(C.stat[2]);
-	b->stats[8] = double(C.stat[3]);
-	b->stats[9] = double(C.stat[4]);
-	b->stats[10] = double(b->nnodes);
-	b->stats[11] = double(C.stat[6]);
-	b->stats[14] = double(C.stat[7]);
-	CT_HIP(hipEventElapsedTime(&ms, b->ev[3], b->ev[6]));
-	b->stats[12] = ms;
-	CT_HIP(hipEventElapsedTime(&ms, b->ev[6], b->ev[4]));
-	b->stats[13] = ms;
+	CT_HIP(hipStreamSynchronize(st));
+	for (ct_box* x : c->boxes) x->host_stale = true;
+	return CT_OK;
+}
+
+int ct_batch_stats(ct_batch_t bt, double* out, int n) {
+	if (!bt || !out) return fail(CT_E_ARG, "ct_batch_stats: bad arguments");
+	for (int q = 0; q < n && q < 15; q++) out[q] = bt->ctx.stats[q];
 	return CT_OK;
 }
 
@@ -2015,16 +2388,17 @@ int ct_tree_read(ct_box_t b, ct_node* nodes, int64_t cap) {
 
 int ct_lists_read(ct_box_t b, int kind, int64_t* off, int32_t* idx, int64_t idx_cap) {
 	if (!b || kind < 0 || kind >= kNumLists || !off) return fail(CT_E_ARG, "ct_lists_read: bad arguments");
-	if (!b->have_lists) return fail(CT_E_STATE, "ct_lists_read: no kick has run");
+	KickCtx* c = static_cast<KickCtx*>(b->self_ctx);
+	if (!c || !c->have_lists) return fail(CT_E_STATE, "ct_lists_read: no kick has run");
 	const int64_t nn = b->nnodes;
-	std::vector<int32_t> head(nn);
-	const Counters& C = b->last_counters;
+	std::vector<int32_t> head(std::max<int64_t>(nn, 1));
+	const Counters& C = c->last_counters;
 	std::vector<Seg> seg(std::max(1, C.seg));
 	std::vector<int32_t> out(std::max(1, C.out[kind]));
 	CT_HIP(hipDeviceSynchronize());
-	CT_HIP(hipMemcpy(head.data(), b->head.p + int64_t(kind) * nn, nn * 4, hipMemcpyDeviceToHost));
-	if (C.seg) CT_HIP(hipMemcpy(seg.data(), b->seg.p, C.seg * sizeof(Seg), hipMemcpyDeviceToHost));
-	if (C.out[kind]) CT_HIP(hipMemcpy(out.data(), b->out[kind].p, int64_t(C.out[kind]) * 4, hipMemcpyDeviceToHost));
+	if (nn) CT_HIP(hipMemcpy(head.data(), c->head.p + int64_t(kind) * nn, nn * 4, hipMemcpyDeviceToHost));
+	if (C.seg) CT_HIP(hipMemcpy(seg.data(), c->seg.p, C.seg * sizeof(Seg), hipMemcpyDeviceToHost));
+	if (C.out[kind]) CT_HIP(hipMemcpy(out.data(), c->out[kind].p, int64_t(C.out[kind]) * 4, hipMemcpyDeviceToHost));
 	int64_t tot = 0;
 	for (int64_t i = 0; i < nn; i++) {
 		off[i] = tot;
@@ -2048,7 +2422,8 @@ int ct_lists_read(ct_box_t b, int kind, int64_t* off, int32_t* idx, int64_t idx_
 
 int ct_kick_stats(ct_box_t b, double* out, int n) {
 	if (!b || !out) return fail(CT_E_ARG, "ct_kick_stats: bad arguments");
-	for (int q = 0; q < n && q < 15; q++) out[q] = b->stats[q];
+	KickCtx* c = static_cast<KickCtx*>(b->self_ctx);
+	for (int q = 0; q < n && q < 15; q++) out[q] = c ? c->stats[q] : 0.0;
 	return CT_OK;
 }
 

@@ -130,6 +130,30 @@ int ct_kick(ct_box_t box, const ct_kick_params* params, ct_kick_return* ret, voi
  * counterpart in the reference, whose kick is FP32 only (src/gravity.cu). */
 int ct_kick_prec(ct_box_t box, const ct_kick_params* params, int precision, ct_kick_return* ret, void* stream);
 
+/* A batch of resident boxes kicked together: every phase of the kick runs as
+ * one launch over all the boxes (one walk per tree level over every box's
+ * sinks, one cell launch, one P2P launch per leaf-size class, ...).  The
+ * reference hands the GPU a list of subtrees of one box in one persistent
+ * launch (kick_workitem, cosmictiger/kick.hpp:188-194; cuda_execute_kicks,
+ * src/kick.cu:508-609); a batch extends that list to subtrees of many boxes.
+ * A box may belong to several batches but must not be kicked concurrently. */
+typedef struct ct_batch* ct_batch_t;
+int ct_batch_create(const ct_box_t* boxes, int nbox, ct_batch_t* out);
+int ct_batch_destroy(ct_batch_t batch);
+
+/* One kick of every box of the batch with params[i] for box i; rets[i]
+ * receives box i's kick_return.  use_graph != 0 replays a captured HIP graph
+ * of the whole kick (captured on the first call after the batch's trees or
+ * arenas change; a replay whose walk overflowed its arenas is redone eagerly). */
+int ct_batch_kick(ct_batch_t batch, const ct_kick_params* params, int precision, ct_kick_return* rets, int use_graph, void* stream);
+
+/* Drift of every box of the batch in one launch (a[i], dt[i], drag[i] for
+ * box i; drag may be NULL), with cuda_drift's arithmetic (src/drift.cu:36-154). */
+int ct_batch_drift(ct_batch_t batch, int rung, const float* a, const float* dt, const double* drag, void* stream);
+
+/* Phase times and event counts of the batch's last kick, as ct_kick_stats. */
+int ct_batch_stats(ct_batch_t batch, double* out, int n);
+
 /* Drift particles of rung `rung` by dt at scale factor a (float arguments as
  * the reference narrows them), with optional drag.  Replaces
  * cuda_drift(char, float a, float dt, ...) (cosmictiger/drift.hpp:54,
