@@ -159,6 +159,43 @@ def ensemble(ct, args):
     print(json.dumps(out), flush=True)
 
 
+def serving(ct, args):
+    """Poisson-arriving requests of 1-10 kick+drift steps on boxes with N
+    log-uniform in [16^3, 192^3]: continuous batching against one box per
+    launch, at 0.5x, 1x and 2x the saturation arrival rate of the batched
+    server (measured first with all requests arriving at once)."""
+    from cosmictiger_amd import serve
+    ct.init(0)
+    ex = serve.GpuExecutor(ct, theta=args.theta, bucket=args.bucket, dt=1e-3)
+    budget = 64 * 2 ** 30
+    nreq = args.serve_requests
+
+    def run(rate, batched):
+        reqs = serve.poisson_requests(nreq, rate, seed=77)
+        srv = serve.Server(ex, hbm_budget=budget, host_threads=min(16, os.cpu_count() or 1), batched=batched)
+        res = srv.run(reqs)
+        res["max_batch"] = srv.max_batch
+        srv.close()
+        return res
+
+    sat = run(1e9, True)
+    rate_sat = nreq / sat["wall_s"]
+    rows = []
+    for f in (0.5, 1.0, 2.0):
+        rate = f * rate_sat
+        b = run(rate, True)
+        s1 = run(rate, False)
+        rows.append({"rate_per_s": rate, "rate_x_saturation": f, "batched": b, "one_box_per_launch": s1})
+        print("serve: rate %.2f/s batched %.3e p-steps/s p50 %.0f ms p99 %.0f ms | one box per launch %.3e p50 %.0f p99 %.0f"
+              % (rate, b["throughput"], b["latency_ms"]["p50"], b["latency_ms"]["p99"], s1["throughput"], s1["latency_ms"]["p50"],
+                 s1["latency_ms"]["p99"]), file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "serving: particle-steps/s and p50/p99 step latency (kick + drift + tree build per step)",
+                      "unit": "particle-steps/s", "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": "%d Poisson requests of 1-10 steps, N log-uniform in [16^3, 192^3], clustered U[0,0.8], "
+                                             "theta %.2f, bucket %d, HBM budget %d GiB" % (nreq, args.theta, args.bucket, budget >> 30)},
+                      "saturation": {"rate_per_s": rate_sat, "run": sat}, "rates": rows}), flush=True)
+
+
 def cpu_baseline(x, y, z, args, h):
     """Oracle kick on a bounded sample of sinks of the same box (CPU, this host)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -209,6 +246,10 @@ def main():
                     help="instead of the throughput line: the 64-box ensemble kicked in one batched, graph-captured "
                          "launch sequence against one box per launch, with the autotuned bucket")
     ap.add_argument("--ensemble-boxes", type=int, default=64)
+    ap.add_argument("--serve", action="store_true",
+                    help="instead of the throughput line: the serving loop (Poisson kick+drift requests, continuous "
+                         "batching) at three arrival rates, against one box per launch")
+    ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pp_kernel_traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -219,6 +260,9 @@ def main():
     import cosmictiger_amd as ct
     if args.ensemble:
         ensemble(ct, args)
+        return
+    if args.serve:
+        serving(ct, args)
         return
     t0 = time.perf_counter()
     x, y, z = make_box(args)

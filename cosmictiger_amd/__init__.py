@@ -31,6 +31,11 @@ CT_E_STATE = -5
 CT_E_TREE = -6
 CT_E_WALK = -7
 
+# where ct_tree_build_ex builds the tree
+TREE_AUTO = 0    # device, host fallback for trees deeper than 63 levels
+TREE_HOST = 1
+TREE_DEVICE = 2
+
 # arithmetic modes of ct_kick_prec
 PREC_F32C = 0  # FP32, FP64 sink accumulation per source tile (the default kick)
 PREC_F32U = 1  # FP32, FP32 running sums
@@ -100,6 +105,7 @@ def _declare(L):
     L.ct_box_create.argtypes = [ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]
     L.ct_box_destroy.argtypes = [_P]
     L.ct_tree_build.argtypes = [_P, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.ct_tree_build_ex.argtypes = [_P, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int]
     L.ct_tree_build_host.argtypes = [ctypes.c_int64, _P, _P, _P, _P, ctypes.c_double, ctypes.c_int, ctypes.c_double, _P,
                                      ctypes.c_int64, _I64P]
     L.ct_kick.argtypes = [_P, ctypes.POINTER(KickParams), ctypes.POINTER(KickReturn), _P]
@@ -120,7 +126,8 @@ def _declare(L):
     L.ct_fmm_device.argtypes = [ctypes.c_int, ctypes.c_int64, _P, _P]
     L.ct_last_error.argtypes = []
     L.ct_last_error.restype = ctypes.c_char_p
-    for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_host", "ct_kick",
+    for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_ex",
+                 "ct_tree_build_host", "ct_kick",
                  "ct_kick_prec",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
                  "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_kick", "ct_batch_drift",
@@ -243,6 +250,11 @@ class Box:
 
     def tree_build(self, theta, bucket, hsoft):
         _check(lib().ct_tree_build(self._h, float(theta), int(bucket), float(hsoft)))
+
+    def tree_build_ex(self, theta, bucket, hsoft, where=TREE_AUTO):
+        """tree_build on the device (TREE_DEVICE), the host (TREE_HOST) or
+        the device with the host as fallback (TREE_AUTO, what tree_build does)."""
+        _check(lib().ct_tree_build_ex(self._h, float(theta), int(bucket), float(hsoft), int(where)))
 
     def kick(self, params, stream=None, precision=None):
         """One kick of the box; precision: None (the default FP32 kick) or one

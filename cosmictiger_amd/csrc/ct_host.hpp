@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
 
@@ -56,7 +57,7 @@ int check_init();
 
 // Source of tree versions: every tree build takes a new one, so that kick
 // contexts can tell that a box's tree changed.
-extern uint64_t g_tree_version;
+extern std::atomic<uint64_t> g_tree_version;
 
 }  // namespace ct
 
@@ -73,6 +74,7 @@ struct ct_box {
 	ct::DBuf<uint32_t> x, y, z;
 	ct::DBuf<float> vx, vy, vz, gx, gy, gz, phi;
 	ct::DBuf<int8_t> rung;
+	ct::DBuf<int32_t> id;  // index of each particle at ct_box_create
 	// tree
 	ct::DBuf<ct::DNode> nodes;
 	ct::DBuf<float> multi, L;
@@ -80,6 +82,7 @@ struct ct_box {
 	ct::DBuf<double> multi64, L64;  // FP64 multipoles and expansions (FP64 kicks only)
 	bool multi64_ready = false;
 	int64_t nnodes = 0;
+	bool host_nodes = false;              // tree.nodes holds the node table (host build)
 	std::vector<int32_t> leaf_class[16];  // non-empty leaves by size class
 	// kick context of single-box kicks (walk arenas, lists of the last kick)
 	void* self_ctx = nullptr;

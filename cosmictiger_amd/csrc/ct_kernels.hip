@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -36,6 +37,7 @@
 #include "ct_expansions.hpp"
 #include "ct_host.hpp"
 #include "ct_tree.hpp"
+#include "ct_treegpu.hpp"
 
 namespace ct {
 
@@ -146,6 +148,26 @@ __device__ __forceinline__ int uniform(int x) {
 	return __builtin_amdgcn_readfirstlane(x);
 }
 
+// A copy of *p held in scalar registers; p must be the same in every lane.
+// (The descriptor tables live in writable global memory, which the compiler
+// will not read with scalar loads by itself.)
+template<class T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+	static_assert(sizeof(T) % 16 == 0, "load_uniform reads 16-byte words");
+	T out;
+	const int4* src = reinterpret_cast<const int4*>(p);
+	int* dst = reinterpret_cast<int*>(&out);
+#pragma unroll
+	for (int i = 0; i < int(sizeof(T) / 16); i++) {
+		const int4 v = src[i];
+		dst[4 * i + 0] = __builtin_amdgcn_readfirstlane(v.x);
+		dst[4 * i + 1] = __builtin_amdgcn_readfirstlane(v.y);
+		dst[4 * i + 2] = __builtin_amdgcn_readfirstlane(v.z);
+		dst[4 * i + 3] = __builtin_amdgcn_readfirstlane(v.w);
+	}
+	return out;
+}
+
 // ---------------------------------------------------------------------------
 // walk
 // ---------------------------------------------------------------------------
@@ -233,7 +255,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 		for (int e = 0; e < nb; e++) {
 			const Act ae = act[b0 + e];
 			const int self = uniform(ae.self);
-			const BoxDesc* B = A.boxes + uniform(ae.box);
+			const BoxDesc Bv = load_uniform(A.boxes + uniform(ae.box));
+			const BoxDesc* B = &Bv;
 			const DNode* nodes = B->nodes;
 			const DNode s = nodes[self];
 			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
@@ -307,7 +330,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 			const Act ae = act[b0 + e];
 			const int self = uniform(ae.self);
 			const int box = uniform(ae.box);
-			const BoxDesc* B = A.boxes + box;
+			const BoxDesc Bv = load_uniform(A.boxes + box);
+			const BoxDesc* B = &Bv;
 			const DNode* nodes = B->nodes;
 			const float thinv = B->theta_inv, hs = B->h;
 			const int db = uniform(ae.db), dn = uniform(ae.dn);
@@ -645,10 +669,22 @@ template<class T>
 __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 	const int lane = threadIdx.x;
 	unsigned long long nev = 0;
-	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
-		const int bx = uniform(find_box(A.node_off, A.nbox, g));
-		const BoxDesc* B = A.boxes + bx;
-		const int i = g - A.node_off[bx];
+	// a contiguous range of nodes per block; the box descriptor is read when
+	// the range enters a new box
+	const int chunk = (A.total + gridDim.x - 1) / gridDim.x;
+	const int g0 = blockIdx.x * chunk, g1 = min(A.total, g0 + chunk);
+	int bx = g0 < g1 ? uniform(find_box(A.node_off, A.nbox, g0)) : 0;
+	BoxDesc Bv = load_uniform(A.boxes + bx);
+	int bend = uniform(A.node_off[bx + 1]), bbeg = uniform(A.node_off[bx]);
+	for (int g = g0; g < g1; g++) {
+		while (g >= bend) {
+			bx++;
+			Bv = load_uniform(A.boxes + bx);
+			bbeg = bend;
+			bend = uniform(A.node_off[bx + 1]);
+		}
+		const BoxDesc* B = &Bv;
+		const int i = g - bbeg;
 		const T* multi = static_cast<const T*>(B->multi);
 		T* Lo = static_cast<T*>(B->L) + int64_t(i) * kLStride;
 		const bool do_phi = B->prm.do_phi;
@@ -689,10 +725,22 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 	__shared__ uint32_t tx[kCellTile], ty[kCellTile], tz[kCellTile];
 	const int lane = threadIdx.x;
 	unsigned long long ncc = 0, ncp = 0;
-	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
-		const int bx = uniform(find_box(A.node_off, A.nbox, g));
-		const BoxDesc* B = A.boxes + bx;
-		const int i = g - A.node_off[bx];
+	// a contiguous range of nodes per block; the box descriptor is read when
+	// the range enters a new box
+	const int chunk = (A.total + gridDim.x - 1) / gridDim.x;
+	const int g0 = blockIdx.x * chunk, g1 = min(A.total, g0 + chunk);
+	int bx = g0 < g1 ? uniform(find_box(A.node_off, A.nbox, g0)) : 0;
+	BoxDesc Bv = load_uniform(A.boxes + bx);
+	int bend = uniform(A.node_off[bx + 1]), bbeg = uniform(A.node_off[bx]);
+	for (int g = g0; g < g1; g++) {
+		while (g >= bend) {
+			bx++;
+			Bv = load_uniform(A.boxes + bx);
+			bbeg = bend;
+			bend = uniform(A.node_off[bx + 1]);
+		}
+		const BoxDesc* B = &Bv;
+		const int i = g - bbeg;
 		const T* multi = static_cast<const T*>(B->multi);
 		const bool do_phi = B->prm.do_phi;
 		const int hcc = uniform(B->head[int64_t(kCC) * B->nnodes + i]);
@@ -906,7 +954,8 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	__shared__ __align__(16) uint32_t ty[P * STR];
 	__shared__ __align__(16) uint32_t tz[P * STR];
 	const int2 le = A.leaves[blockIdx.x];
-	const BoxDesc* B = A.boxes + uniform(le.x);
+	const BoxDesc Bv = load_uniform(A.boxes + uniform(le.x));
+	const BoxDesc* B = &Bv;
 	const int self = uniform(le.y);
 	const DNode* nodes = B->nodes;
 	const uint32_t* px = B->x;
@@ -1091,20 +1140,21 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 // whatever the leaf size; the 4 results of a pair are added to the sink's FP64
 // sums in LDS.  L2P and the kick then run a sink per lane.
 template<int PR>
-__global__ __launch_bounds__(64) void k_kick(LeafArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_kick(LeafArgs A) {
 	using T = typename Prec<PR>::T;
 	using Acc = typename Prec<PR>::A;
 	__shared__ Acc acc[4 * kMaxLeafSinks];
 	const int li = blockIdx.x;
 	const int2 le = A.leaves[li];
-	const BoxDesc* B = A.boxes + uniform(le.x);
+	const BoxDesc Bv = load_uniform(A.boxes + uniform(le.x));
+	const BoxDesc* B = &Bv;
 	const int self = uniform(le.y);
 	const T* multi = static_cast<const T*>(B->multi);
 	const DNode s = B->nodes[self];
 	const int pbeg = s.pbegin;
 	const int nsink = s.pend - s.pbegin;
 	const int lane = threadIdx.x;
-	const ct_kick_params P_ = B->prm;
+	const ct_kick_params& P_ = Bv.prm;
 	const uint32_t* px = B->x;
 	const uint32_t* py = B->y;
 	const uint32_t* pz = B->z;
@@ -1403,7 +1453,7 @@ __global__ void k_fmm(int op, int64_t n, const double* __restrict__ in, double* 
 // ---------------------------------------------------------------------------
 
 thread_local std::string g_err;
-uint64_t g_tree_version = 0;
+std::atomic<uint64_t> g_tree_version{0};
 
 struct EwaldHost {
 	std::vector<float> rimg, fh, fc;
@@ -1502,9 +1552,10 @@ int upload_box(ct_box* b) {
 	int rc;
 	if ((rc = b->x.grow(n)) || (rc = b->y.grow(n)) || (rc = b->z.grow(n)) || (rc = b->vx.grow(n)) || (rc = b->vy.grow(n)) ||
 			(rc = b->vz.grow(n)) || (rc = b->rung.grow(n)) || (rc = b->gx.grow(n)) || (rc = b->gy.grow(n)) || (rc = b->gz.grow(n)) ||
-			(rc = b->phi.grow(n)))
+			(rc = b->phi.grow(n)) || (rc = b->id.grow(n)))
 		return rc;
 	if (n) {
+		CT_HIP(hipMemcpy(b->id.p, b->hp.id.data(), n * 4, hipMemcpyHostToDevice));
 		CT_HIP(hipMemcpy(b->x.p, b->hp.x.data(), n * 4, hipMemcpyHostToDevice));
 		CT_HIP(hipMemcpy(b->y.p, b->hp.y.data(), n * 4, hipMemcpyHostToDevice));
 		CT_HIP(hipMemcpy(b->z.p, b->hp.z.data(), n * 4, hipMemcpyHostToDevice));
@@ -1523,7 +1574,6 @@ int upload_box(ct_box* b) {
 int download_particles(ct_box* b) {
 	const int64_t n = b->hp.n;
 	if (!b->host_stale || n == 0) return CT_OK;
-	CT_HIP(hipDeviceSynchronize());
 	CT_HIP(hipMemcpy(b->hp.x.data(), b->x.p, n * 4, hipMemcpyDeviceToHost));
 	CT_HIP(hipMemcpy(b->hp.y.data(), b->y.p, n * 4, hipMemcpyDeviceToHost));
 	CT_HIP(hipMemcpy(b->hp.z.data(), b->z.p, n * 4, hipMemcpyDeviceToHost));
@@ -1531,6 +1581,7 @@ int download_particles(ct_box* b) {
 	CT_HIP(hipMemcpy(b->hp.vy.data(), b->vy.p, n * 4, hipMemcpyDeviceToHost));
 	CT_HIP(hipMemcpy(b->hp.vz.data(), b->vz.p, n * 4, hipMemcpyDeviceToHost));
 	CT_HIP(hipMemcpy(b->hp.rung.data(), b->rung.p, n, hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpy(b->hp.id.data(), b->id.p, n * 4, hipMemcpyDeviceToHost));
 	b->host_stale = false;
 	return CT_OK;
 }
@@ -1542,6 +1593,7 @@ int ensure_fp64(ct_box* b) {
 	int rc;
 	if ((rc = b->L64.grow(std::max<int64_t>(1, nn) * kLStride))) return rc;
 	if (b->multi64_ready) return CT_OK;
+	if (!b->host_nodes) return fail(CT_E_STATE, "FP64 multipoles missing");
 	if ((rc = b->multi64.grow(std::max<int64_t>(1, nn) * kMultiStride))) return rc;
 	std::vector<double> mp(nn * kMultiStride, 0.0);
 	for (int64_t i = 0; i < nn; i++) {
@@ -2131,6 +2183,7 @@ int ct_box_destroy(ct_box_t b) {
 	b->gz.release();
 	b->phi.release();
 	b->rung.release();
+	b->id.release();
 	b->nodes.release();
 	b->multi.release();
 	b->L.release();
@@ -2142,9 +2195,46 @@ int ct_box_destroy(ct_box_t b) {
 }
 
 int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
+	return ct_tree_build_ex(b, theta, bucket, hsoft, CT_TREE_AUTO);
+}
+
+int ct_tree_build_ex(ct_box_t b, double theta, int bucket, double hsoft, int where) {
 	int rc = check_init();
 	if (rc) return rc;
-	if (!b || !(theta > 0.0) || bucket < 1 || bucket > kMaxLeafSinks || hsoft < 0.0) return fail(CT_E_ARG, "ct_tree_build: bad arguments");
+	if (!b || !(theta > 0.0) || bucket < 1 || bucket > kMaxLeafSinks || hsoft < 0.0 ||
+			(where != CT_TREE_AUTO && where != CT_TREE_HOST && where != CT_TREE_DEVICE))
+		return fail(CT_E_ARG, "ct_tree_build: bad arguments");
+	if (where != CT_TREE_HOST) {
+		GpuTreeResult R;
+		rc = gpu_tree_build(b, theta, bucket, hsoft, &R);
+		if (rc == CT_OK) {
+			std::vector<int32_t> cls[kNumLeafClasses];
+			for (size_t q = 0; q < R.leaf_id.size(); q++) {
+				const int np = R.leaf_pend[q] - R.leaf_pbegin[q];
+				if (np == 0) continue;
+				if (np > kMaxLeafSinks) return fail(CT_E_TREE, "ct_tree_build: leaf larger than 256 particles");
+				cls[leaf_class(np)].push_back(R.leaf_id[q]);
+			}
+			b->tree.nodes.clear();
+			b->tree.level_off = R.level_off;
+			b->tree.max_depth = R.max_depth;
+			b->tree.nleaves = int64_t(R.leaf_id.size());
+			b->host_nodes = false;
+			b->theta = theta;
+			b->bucket = bucket;
+			b->hsoft = hsoft;
+			b->nnodes = R.nnodes;
+			if ((rc = b->pp.grow(std::max<int64_t>(1, b->hp.n)))) return rc;
+			for (int c = 0; c < kNumLeafClasses; c++) b->leaf_class[c] = std::move(cls[c]);
+			b->have_tree = true;
+			b->multi64_ready = true;  // the device build writes the FP64 multipoles too
+			b->host_stale = true;     // particles were permuted on the device
+			b->tree_version = ++g_tree_version;
+			return CT_OK;
+		}
+		if (where == CT_TREE_DEVICE || rc != CT_E_TREE) return rc;
+		// deeper than the 63-bit key: the host build takes it
+	}
 	if ((rc = download_particles(b))) return rc;
 	TreeBuildParams prm;
 	prm.theta = theta;
@@ -2179,7 +2269,6 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 			cls[leaf_class(h.pend - h.pbegin)].push_back(int32_t(i));
 		}
 	}
-	(void) hipDeviceSynchronize();
 	if ((rc = b->nodes.grow(nn)) || (rc = b->multi.grow(nn * kMultiStride)) || (rc = b->L.grow(nn * kLStride)) ||
 			(rc = b->pp.grow(std::max<int64_t>(1, b->hp.n))))
 		return rc;
@@ -2187,6 +2276,7 @@ int ct_tree_build(ct_box_t b, double theta, int bucket, double hsoft) {
 	CT_HIP(hipMemcpy(b->multi.p, mp.data(), mp.size() * sizeof(float), hipMemcpyHostToDevice));
 	if ((rc = upload_box(b))) return rc;
 	b->tree = std::move(T);
+	b->host_nodes = true;
 	b->theta = theta;
 	b->bucket = bucket;
 	b->hsoft = hsoft;
@@ -2244,7 +2334,8 @@ int ct_batch_kick(ct_batch_t bt, const ct_kick_params* params, int precision, ct
 	int rc = check_init();
 	if (rc) return rc;
 	if (!bt || !params) return fail(CT_E_ARG, "ct_batch_kick: bad arguments");
-	return ctx_kick(&bt->ctx, params, precision, rets, use_graph != 0, reinterpret_cast<hipStream_t>(stream));
+	hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : bt->ctx.own;
+	return ctx_kick(&bt->ctx, params, precision, rets, use_graph != 0, st);
 }
 
 int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, const double* drag, void* stream) {
@@ -2262,7 +2353,7 @@ int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, con
 		d.do_drag = dg != 0.0;
 		d.drift_rung = rung;
 	}
-	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+	hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : c->own;
 	CT_HIP(hipMemcpyAsync(c->desc.p, c->hdesc.data(), c->hdesc.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, st));
 	if (c->total_parts > 0)
 		hipLaunchKernelGGL(k_drift_batch, dim3((c->total_parts + 255) / 256), dim3(256), 0, st, c->desc.p, c->part_off.p, c->nbox(),
@@ -2324,7 +2415,7 @@ int ct_box_read(ct_box_t b, uint32_t* x, uint32_t* y, uint32_t* z, float* vel, i
 	if (gy) CT_HIP(hipMemcpy(gy, b->gy.p, n * 4, hipMemcpyDeviceToHost));
 	if (gz) CT_HIP(hipMemcpy(gz, b->gz.p, n * 4, hipMemcpyDeviceToHost));
 	if (phi) CT_HIP(hipMemcpy(phi, b->phi.p, n * 4, hipMemcpyDeviceToHost));
-	if (id) std::memcpy(id, b->hp.id.data(), n * sizeof(int32_t));
+	if (id) CT_HIP(hipMemcpy(id, b->id.p, n * 4, hipMemcpyDeviceToHost));
 	return CT_OK;
 }
 
@@ -2382,7 +2473,37 @@ int ct_tree_read(ct_box_t b, ct_node* nodes, int64_t cap) {
 	if (!b || !nodes) return fail(CT_E_ARG, "ct_tree_read: bad arguments");
 	if (!b->have_tree) return fail(CT_E_STATE, "ct_tree_read: no tree");
 	if (cap < b->nnodes) return fail(CT_E_ARG, "ct_tree_read: capacity too small");
-	for (int64_t i = 0; i < b->nnodes; i++) host_node_to_abi(b->tree.nodes[i], nodes[i]);
+	if (b->host_nodes) {
+		for (int64_t i = 0; i < b->nnodes; i++) host_node_to_abi(b->tree.nodes[i], nodes[i]);
+		return CT_OK;
+	}
+	// built on the device: read the node table and multipoles back
+	const int64_t nn = b->nnodes;
+	std::vector<DNode> dn(nn);
+	std::vector<float> mp(nn * kMultiStride);
+	CT_HIP(hipDeviceSynchronize());
+	if (nn) {
+		CT_HIP(hipMemcpy(dn.data(), b->nodes.p, nn * sizeof(DNode), hipMemcpyDeviceToHost));
+		CT_HIP(hipMemcpy(mp.data(), b->multi.p, mp.size() * sizeof(float), hipMemcpyDeviceToHost));
+	}
+	for (int64_t i = 0; i < nn; i++) {
+		const DNode& d = dn[i];
+		ct_node& o = nodes[i];
+		for (int q = 0; q < 3; q++) {
+			o.box_begin[q] = d.bb[q];
+			o.box_end[q] = d.be[q];
+			o.pos[q] = d.pos[q];
+		}
+		o.child[0] = d.child[0];
+		o.child[1] = d.child[1];
+		o.parent = d.parent;
+		o.pbegin = d.pbegin;
+		o.pend = d.pend;
+		o.radius = d.radius;
+		o.leaf = d.flags & 1;
+		o.depth = d.flags >> 1;
+		std::memcpy(o.multi, &mp[i * kMultiStride], sizeof(o.multi));
+	}
 	return CT_OK;
 }
 

@@ -180,6 +180,8 @@ class Server:
         self.pool = cf.ThreadPoolExecutor(max_workers=host_threads)
         self.batched = batched
         self._cancel = set()
+        self.max_used = 0  # largest admitted footprint seen in the last run
+        self.max_batch = 0  # most boxes stepped together in the last run
 
     def cancel(self, rid):
         """Cancel a request: it takes no further step and its box is freed."""
@@ -208,6 +210,7 @@ class Server:
                 r = pending.pop(0)
                 r.admitted_at = now()
                 used += footprint(r.n)
+                self.max_used = max(self.max_used, used)
                 active[r.rid] = {"req": r, "state": None, "future": self.pool.submit(self.ex.create, r), "ready_at": r.arrival}
             # cancellations
             for rid in list(active):
@@ -240,6 +243,7 @@ class Server:
                 else:
                     time.sleep(0.0005)
                 continue
+            self.max_batch = max(self.max_batch, len(ready))
             if self.batched:
                 self.ex.step([a["state"] for a in ready])
             else:

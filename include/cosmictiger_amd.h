@@ -105,6 +105,18 @@ int ct_box_destroy(ct_box_t box);
  * Replaces tree_create (cosmictiger/tree.hpp:207, src/tree.cpp:273). */
 int ct_tree_build(ct_box_t box, double theta, int bucket, double hsoft);
 
+/* Where ct_tree_build_ex builds the tree: AUTO = on the device (particles
+ * sorted by Morton key, which is the order of the reference's bisections),
+ * falling back to the host for trees deeper than 63 levels; HOST = the host
+ * build (in-place partition as src/particles.cpp:979-1018); DEVICE = device
+ * only (CT_E_TREE when too deep).  Both give the same nodes (boxes, centres,
+ * radii, particle ranges, topology); the order of particles inside a leaf
+ * differs. */
+#define CT_TREE_AUTO 0
+#define CT_TREE_HOST 1
+#define CT_TREE_DEVICE 2
+int ct_tree_build_ex(ct_box_t box, double theta, int bucket, double hsoft, int where);
+
 /* Host-only tree build over caller arrays (no device needed): the same build
  * as ct_tree_build.  Permutes x, y, z in place; perm[i] receives the original
  * index of the particle now at i.  Writes up to node_cap nodes (breadth-first,

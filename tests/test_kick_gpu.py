@@ -211,3 +211,32 @@ def test_kick_precision_modes(gpu, oc, prec, rms_bar, kind, n, theta):
         assert np.sqrt(np.mean(dphi ** 2)) <= 2.4e-7 * np.max(np.abs(res["oracle"]["phi"]))
     else:
         assert np.sqrt(np.mean(dphi ** 2)) <= 1e-5  # GM N = 1
+
+
+@pytest.mark.parametrize("kind,n,theta,bucket", [("uniform", 60000, 0.7, 128), ("clustered", 80000, 0.55, 64),
+                                                  ("clustered", 30000, 0.4, 32), ("uniform", 1, 0.7, 128), ("uniform", 0, 0.7, 128),
+                                                  ("uniform", 300, 0.55, 2)])
+def test_device_tree_equals_host_tree(gpu, kind, n, theta, bucket):
+    """The device tree build gives the host build's nodes: boxes, centres,
+    radii, particle ranges, leaf flags, depths and topology bit for bit,
+    multipoles to FP64 rounding, the same particles in every leaf."""
+    x, y, z = _box(kind, n, 31)
+    h = gpu.softening(max(n, 1))
+    a = gpu.Box(x, y, z)
+    b = gpu.Box(x, y, z)
+    a.tree_build_ex(theta, bucket, h, gpu.TREE_HOST)
+    b.tree_build_ex(theta, bucket, h, gpu.TREE_DEVICE)
+    na, nb = a.nodes(), b.nodes()
+    assert len(na) == len(nb)
+    for f in ("box_begin", "box_end", "child", "parent", "pbegin", "pend", "pos", "leaf", "depth"):
+        assert np.array_equal(na[f], nb[f]), f
+    assert np.array_equal(na["radius"].view(np.uint32), nb["radius"].view(np.uint32))
+    scale = np.maximum(np.abs(na["multi"][:, :1]), 1e-30) * (1000.0 * np.maximum(na["radius"], 1e-9))[:, None] ** 6
+    assert np.max(np.abs(na["multi"] - nb["multi"]) / np.maximum(scale, 1e-30)) <= 1e-6
+    ida, idb = a.read()["id"], b.read()["id"]
+    for nd in na[na["leaf"] == 1]:
+        sa = np.sort(ida[nd["pbegin"]:nd["pend"]])
+        sb = np.sort(idb[nd["pbegin"]:nd["pend"]])
+        assert np.array_equal(sa, sb)
+    a.close()
+    b.close()
