@@ -99,6 +99,35 @@ _P = ctypes.c_void_p
 _I64P = ctypes.POINTER(ctypes.c_int64)
 
 
+class StepParams(ctypes.Structure):
+    """Arguments of one call of the hierarchical step (kick_step_hierarchical,
+    src/driver.cpp:382-593), ct_step_params of the C-ABI."""
+    _fields_ = [("minrung", ctypes.c_int), ("max_rung", ctypes.c_int), ("minrung0", ctypes.c_int), ("scale", ctypes.c_double),
+                ("tau", ctypes.c_double), ("t0", ctypes.c_double), ("theta", ctypes.c_double), ("bucket", ctypes.c_int),
+                ("hsoft", ctypes.c_double), ("GM", ctypes.c_double), ("eta", ctypes.c_double), ("max_dt", ctypes.c_double),
+                ("sign", ctypes.c_double), ("do_phi", ctypes.c_int), ("nocrop", ctypes.c_int), ("drag", ctypes.c_double),
+                ("save_force", ctypes.c_int), ("precision", ctypes.c_int), ("tree_where", ctypes.c_int)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.scale = 1.0
+        self.t0 = 1.0
+        self.theta = 0.7
+        self.bucket = 128
+        self.eta = 0.25 / np.sqrt(SELF_PHI)
+        self.max_dt = 1e30
+        self.sign = 1.0
+        self.do_phi = 1
+        self.save_force = 1
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+
+class StepReturn(ctypes.Structure):
+    _fields_ = [("kr", KickReturn), ("minrung", ctypes.c_int), ("nkicks", ctypes.c_int), ("clip_top", ctypes.c_int),
+                ("tckin", ctypes.c_double), ("parts_processed", ctypes.c_double)]
+
+
 def _declare(L):
     L.ct_init.argtypes = [ctypes.c_int]
     L.ct_ewald_init.argtypes = []
@@ -122,6 +151,17 @@ def _declare(L):
     L.ct_batch_kick.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P]
     L.ct_batch_drift.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
     L.ct_batch_stats.argtypes = [_P, _P, ctypes.c_int]
+    L.ct_box_sort_by_rung.argtypes = [_P, ctypes.c_int, _I64P]
+    L.ct_box_current_range.argtypes = [_P, _I64P, _I64P]
+    L.ct_box_push_rungs.argtypes = [_P]
+    L.ct_box_pop_rungs.argtypes = [_P]
+    L.ct_box_set_minrung.argtypes = [_P, ctypes.c_int]
+    L.ct_box_rung_counts.argtypes = [_P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.ct_step_hierarchical.argtypes = [_P, ctypes.POINTER(StepParams), ctypes.POINTER(StepReturn), _P]
+    L.ct_time_inc.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    L.ct_time_inc.restype = ctypes.c_uint64
+    L.ct_time_min_rung.argtypes = [ctypes.c_uint64]
+    L.ct_time_min_rung.restype = ctypes.c_int
     L.ct_fmm_host.argtypes = [ctypes.c_int, _P, _P]
     L.ct_fmm_device.argtypes = [ctypes.c_int, ctypes.c_int64, _P, _P]
     L.ct_last_error.argtypes = []
@@ -131,7 +171,8 @@ def _declare(L):
                  "ct_kick_prec",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
                  "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_kick", "ct_batch_drift",
-                 "ct_batch_stats"):
+                 "ct_batch_stats", "ct_box_sort_by_rung", "ct_box_current_range", "ct_box_push_rungs", "ct_box_pop_rungs",
+                 "ct_box_set_minrung", "ct_box_rung_counts", "ct_step_hierarchical"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -267,7 +308,46 @@ class Box:
         return ret
 
     def drift(self, rung, a, dt, drag=0.0, stream=None):
+        """cuda_drift over the active range (src/drift.cu:156-195)."""
         _check(lib().ct_drift(self._h, int(rung), float(a), float(dt), float(drag), stream))
+
+    # ---- active range and rungs (src/particles.cpp:218-235,583-649,1020-1068)
+
+    def sort_by_rung(self, min_rung):
+        """particles_sort_by_rung: particles of rung >= min_rung to the end of
+        the active range, which then starts at the returned index."""
+        rb = ctypes.c_int64()
+        _check(lib().ct_box_sort_by_rung(self._h, int(min_rung), ctypes.byref(rb)))
+        return rb.value
+
+    def current_range(self):
+        """particles_current_range: (first, second) of the active range."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().ct_box_current_range(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def push_rungs(self):
+        _check(lib().ct_box_push_rungs(self._h))
+
+    def pop_rungs(self):
+        _check(lib().ct_box_pop_rungs(self._h))
+
+    def set_minrung(self, min_rung):
+        _check(lib().ct_box_set_minrung(self._h, int(min_rung)))
+
+    def rung_counts(self):
+        """particles_rung_counts: counts[r] for r up to the highest rung present."""
+        c = np.zeros(128, np.int64)
+        nr = ctypes.c_int()
+        _check(lib().ct_box_rung_counts(self._h, _ptr(c), 128, ctypes.byref(nr)))
+        return c[:nr.value].copy()
+
+    def step_hierarchical(self, params, stream=None):
+        """One call of the kick ladder (kick_step_hierarchical,
+        src/driver.cpp:382-593).  Returns a StepReturn."""
+        out = StepReturn()
+        _check(lib().ct_step_hierarchical(self._h, ctypes.byref(params), ctypes.byref(out), stream))
+        return out
 
     def size(self):
         n = ctypes.c_int64()
@@ -387,6 +467,22 @@ def default_kick_params(theta, h, GM, eta=None, do_phi=True, save_force=True):
     p.top = True
     p.do_phi = do_phi
     return p
+
+
+def time_inc(t, max_rung):
+    """inc() of cosmictiger/time.hpp:28-33 (the C-ABI's ct_time_inc)."""
+    if max_rung > 0:
+        t = (t + (1 << (64 - max_rung))) & ((1 << 64) - 1)
+    return t
+
+
+def time_min_rung(t):
+    """min_rung() of cosmictiger/time.hpp:35-43 (the C-ABI's ct_time_min_rung)."""
+    r = 64
+    while (t & 1) == 0 and r != 0:
+        r -= 1
+        t >>= 1
+    return r
 
 
 def softening(n):

@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 HOST_ARCH = ["-mavx2", "-mfma"]
-HIP_SOURCES = ["ct_kernels.hip", "ct_direct.hip", "ct_treegpu.hip"]
+HIP_SOURCES = ["ct_kernels.hip", "ct_direct.hip", "ct_treegpu.hip", "ct_rungs.hip"]
 
 
 def run(cmd):
@@ -35,7 +35,10 @@ def build(regenerate=True):
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
     tree_o = os.path.join(build_dir, "ct_tree.o")
-    run([HIPCC] + COMMON + HOST_ARCH + ["-x", "c++", "-c", os.path.join(CSRC, "ct_tree.cpp"), "-o", tree_o])
+    # no FMA contraction in the host tree build: the device build
+    # (ct_treegpu.hip) repeats its arithmetic operation for operation
+    run([HIPCC] + COMMON + HOST_ARCH + ["-ffp-contract=off", "-x", "c++", "-c", os.path.join(CSRC, "ct_tree.cpp"), "-o",
+                                        tree_o])
     objs.append(tree_o)
     for src in HIP_SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")

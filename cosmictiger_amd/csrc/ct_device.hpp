@@ -67,6 +67,7 @@ struct __align__(16) BoxDesc {
 	double drag_c;    // 1 / (1 + drag dt)
 	int32_t drift_rung;
 	int32_t do_drag;
+	int32_t drift_first;  // start of the box's active range
 };
 
 // Output list segment: entries out[begin .. begin+len), next segment of the

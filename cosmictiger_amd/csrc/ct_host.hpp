@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/cosmictiger_amd.h"
 #include "ct_device.hpp"
@@ -86,4 +87,9 @@ struct ct_box {
 	std::vector<int32_t> leaf_class[16];  // non-empty leaves by size class
 	// kick context of single-box kicks (walk arenas, lists of the last kick)
 	void* self_ctx = nullptr;
+	// active range [rung_begin, n): trees, kicks and drifts cover it
+	// (rung_begin of src/particles.cpp:1020-1076); the stack of
+	// particles_push_rungs / particles_pop_rungs
+	int64_t rung_begin = 0;
+	std::vector<int64_t> rung_stack;
 };

@@ -669,22 +669,13 @@ template<class T>
 __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 	const int lane = threadIdx.x;
 	unsigned long long nev = 0;
-	// a contiguous range of nodes per block; the box descriptor is read when
-	// the range enters a new box
-	const int chunk = (A.total + gridDim.x - 1) / gridDim.x;
-	const int g0 = blockIdx.x * chunk, g1 = min(A.total, g0 + chunk);
-	int bx = g0 < g1 ? uniform(find_box(A.node_off, A.nbox, g0)) : 0;
-	BoxDesc Bv = load_uniform(A.boxes + bx);
-	int bend = uniform(A.node_off[bx + 1]), bbeg = uniform(A.node_off[bx]);
-	for (int g = g0; g < g1; g++) {
-		while (g >= bend) {
-			bx++;
-			Bv = load_uniform(A.boxes + bx);
-			bbeg = bend;
-			bend = uniform(A.node_off[bx + 1]);
-		}
+	// one node per workgroup (the dispatcher balances the very uneven list
+	// lengths); the box descriptor is kept in scalar registers
+	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
+		const int bx = uniform(find_box(A.node_off, A.nbox, g));
+		const BoxDesc Bv = load_uniform(A.boxes + bx);
 		const BoxDesc* B = &Bv;
-		const int i = g - bbeg;
+		const int i = g - uniform(A.node_off[bx]);
 		const T* multi = static_cast<const T*>(B->multi);
 		T* Lo = static_cast<T*>(B->L) + int64_t(i) * kLStride;
 		const bool do_phi = B->prm.do_phi;
@@ -717,34 +708,27 @@ __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 	if (lane == 0 && nev) atomicAdd(&A.cnt->stat[2], nev);
 }
 
-// Direct CC = M2L (src/gravity.cu:44-73), a source cell per lane, and CP = P2L
-// (src/gravity.cu:75-155), the particles of the CP leaves staged through LDS
-// and a source particle per lane; one wave per sink node.
-template<class T>
+// Direct CC = M2L (src/gravity.cu:44-73), a source cell per lane (CP =
+// false), and CP = P2L (src/gravity.cu:75-155), the particles of the CP
+// leaves staged through LDS and a source particle per lane (CP = true); one
+// wave per sink node.  Two launches, so that the P2L loop is not held to the
+// register budget of M2L.
+template<class T, bool CP>
 __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 	__shared__ uint32_t tx[kCellTile], ty[kCellTile], tz[kCellTile];
 	const int lane = threadIdx.x;
 	unsigned long long ncc = 0, ncp = 0;
-	// a contiguous range of nodes per block; the box descriptor is read when
-	// the range enters a new box
-	const int chunk = (A.total + gridDim.x - 1) / gridDim.x;
-	const int g0 = blockIdx.x * chunk, g1 = min(A.total, g0 + chunk);
-	int bx = g0 < g1 ? uniform(find_box(A.node_off, A.nbox, g0)) : 0;
-	BoxDesc Bv = load_uniform(A.boxes + bx);
-	int bend = uniform(A.node_off[bx + 1]), bbeg = uniform(A.node_off[bx]);
-	for (int g = g0; g < g1; g++) {
-		while (g >= bend) {
-			bx++;
-			Bv = load_uniform(A.boxes + bx);
-			bbeg = bend;
-			bend = uniform(A.node_off[bx + 1]);
-		}
+	// one node per workgroup (the dispatcher balances the very uneven list
+	// lengths); the box descriptor is kept in scalar registers
+	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
+		const int bx = uniform(find_box(A.node_off, A.nbox, g));
+		const BoxDesc Bv = load_uniform(A.boxes + bx);
 		const BoxDesc* B = &Bv;
-		const int i = g - bbeg;
+		const int i = g - uniform(A.node_off[bx]);
 		const T* multi = static_cast<const T*>(B->multi);
 		const bool do_phi = B->prm.do_phi;
-		const int hcc = uniform(B->head[int64_t(kCC) * B->nnodes + i]);
-		const int hcp = uniform(B->head[int64_t(kCP) * B->nnodes + i]);
+		const int hcc = CP ? -1 : uniform(B->head[int64_t(kCC) * B->nnodes + i]);
+		const int hcp = CP ? uniform(B->head[int64_t(kCP) * B->nnodes + i]) : -1;
 		if (hcc < 0 && hcp < 0) continue;
 		const uint32_t sp0 = uniform(int(B->nodes[i].pos[0])), sp1 = uniform(int(B->nodes[i].pos[1])), sp2 = uniform(int(B->nodes[i].pos[2]));
 		T L[50];
@@ -876,14 +860,37 @@ struct LeafArgs {
 
 constexpr int kPPTile = 640;  // sources per LDS tile (multiple of 16; ~8 KB per block)
 
+// Positions in the P2P kernel are offsets from the sink leaf's centre in
+// units of 2^-32, converted once per particle: int32(p - c) is exact, and its
+// float is exact below 2^24 (a 2^-8 box) and off by at most a unit farther
+// out, below the rounding of Delta itself.
+template<class T>
+__device__ __forceinline__ void load4(const T* p, T& a, T& b, T& c, T& d);
+template<>
+__device__ __forceinline__ void load4<float>(const float* p, float& a, float& b, float& c, float& d) {
+	const float4 v = *reinterpret_cast<const float4*>(p);
+	a = v.x;
+	b = v.y;
+	c = v.z;
+	d = v.w;
+}
+template<>
+__device__ __forceinline__ void load4<double>(const double* p, double& a, double& b, double& c, double& d) {
+	const double2 v = *reinterpret_cast<const double2*>(p);
+	const double2 w = *reinterpret_cast<const double2*>(p + 2);
+	a = v.x;
+	b = v.y;
+	c = w.x;
+	d = w.y;
+}
+
 // One softened pair in integer units (Delta = sink - source in units of 2^-32),
 // g -= Delta/r^3, phi -= 1/r; Ferrers n=4 inside h (cosmictiger/gravity.hpp:76-96).
 template<class T>
-__device__ __forceinline__ void pp_pair(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, T h2, T hinv, T h2inv,
-		T h3inv, T& fx, T& fy, T& fz, T& pt) {
-	const T dx = T(int32_t(sx - X));
-	const T dy = T(int32_t(sy - Y));
-	const T dz = T(int32_t(sz - Z));
+__device__ __forceinline__ void pp_pair(T sx, T sy, T sz, T X, T Y, T Z, T h2, T hinv, T h2inv, T h3inv, T& fx, T& fy, T& fz, T& pt) {
+	const T dx = sx - X;
+	const T dy = sy - Y;
+	const T dz = sz - Z;
 	const T r2 = ctfmm::ct_fma(dx, dx, ctfmm::ct_fma(dy, dy, dz * dz));
 	T rinv = ctfmm::ct_rsqrt(r2);
 	T r3 = rinv * rinv * rinv;
@@ -912,11 +919,10 @@ __device__ __forceinline__ void pp_pair(uint32_t sx, uint32_t sy, uint32_t sz, u
 // The same pair without the softening test, for source leaves that cannot
 // come within h of any sink (no pair of the two leaves has r <= h).
 template<class T>
-__device__ __forceinline__ void pp_pair_far(uint32_t sx, uint32_t sy, uint32_t sz, uint32_t X, uint32_t Y, uint32_t Z, T& fx, T& fy, T& fz,
-		T& pt) {
-	const T dx = T(int32_t(sx - X));
-	const T dy = T(int32_t(sy - Y));
-	const T dz = T(int32_t(sz - Z));
+__device__ __forceinline__ void pp_pair_far(T sx, T sy, T sz, T X, T Y, T Z, T& fx, T& fy, T& fz, T& pt) {
+	const T dx = sx - X;
+	const T dy = sy - Y;
+	const T dz = sz - Z;
 	const T r2 = ctfmm::ct_fma(dx, dx, ctfmm::ct_fma(dy, dy, dz * dz));
 	const T rinv = ctfmm::ct_rsqrt(r2);
 	const T r3 = rinv * rinv * rinv;
@@ -950,9 +956,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	using Acc = typename Prec<PR>::A;
 	constexpr int S = 64 / P;
 	constexpr int STR = kPPTile / P + 16;
-	__shared__ __align__(16) uint32_t tx[P * STR];
-	__shared__ __align__(16) uint32_t ty[P * STR];
-	__shared__ __align__(16) uint32_t tz[P * STR];
+	__shared__ __align__(16) T tx[P * STR];
+	__shared__ __align__(16) T ty[P * STR];
+	__shared__ __align__(16) T tz[P * STR];
 	const int2 le = A.leaves[blockIdx.x];
 	const BoxDesc Bv = load_uniform(A.boxes + uniform(le.x));
 	const BoxDesc* B = &Bv;
@@ -963,9 +969,11 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const uint32_t* pz = B->z;
 	const int pbeg = uniform(nodes[self].pbegin);
 	const int nsink = uniform(nodes[self].pend) - pbeg;
+	// the sink leaf's centre: origin of the offsets
+	const uint32_t cs0 = uniform(int(nodes[self].pos[0])), cs1 = uniform(int(nodes[self].pos[1])), cs2 = uniform(int(nodes[self].pos[2]));
 	const int lane = threadIdx.x;
 	const int slot = lane % S, phase = lane / S;
-	uint32_t sx[G], sy[G], sz[G];
+	T sx[G], sy[G], sz[G];
 	T fx[G], fy[G], fz[G], pt[G];
 	Acc gx[G], gy[G], gz[G], ph[G];
 	const float hf = B->h;
@@ -973,9 +981,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	for (int g = 0; g < G; g++) {
 		int k = g * S + slot;
 		if (k >= nsink) k = 0;
-		sx[g] = px[pbeg + k];
-		sy[g] = py[pbeg + k];
-		sz[g] = pz[pbeg + k];
+		sx[g] = T(int32_t(px[pbeg + k] - cs0));
+		sy[g] = T(int32_t(py[pbeg + k] - cs1));
+		sz[g] = T(int32_t(pz[pbeg + k] - cs2));
 		fx[g] = fy[g] = fz[g] = pt[g] = T(0);
 		gx[g] = gy[g] = gz[g] = Acc(0);
 		ph[g] = phase == 0 ? Acc(PR == kF64 ? double(kSelfPhi) / double(hf) : double(kSelfPhi / hf)) : Acc(0);
@@ -988,9 +996,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const T ih3inv = ih2inv * ihinv;
 	unsigned long long nsrc = 0;
 	int tn = 0;
-	const uint32_t* bx = tx + phase * STR;
-	const uint32_t* by = ty + phase * STR;
-	const uint32_t* bz = tz + phase * STR;
+	const T* bx = tx + phase * STR;
+	const T* by = ty + phase * STR;
+	const T* bz = tz + phase * STR;
 	// near = true: a tile of sources that may lie within h of a sink (checked
 	// pairs with the softening branch); near = false: the branch-free pair.
 	auto flush = [&](auto near_tag) __attribute__((always_inline)) {
@@ -1002,11 +1010,14 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 #pragma unroll
 		for (int g = 0; g < G; g++) {
 			T ax = fx[g], ay = fy[g], az = fz[g], ap = pt[g];
-			const uint32_t X0 = sx[g], Y0 = sy[g], Z0 = sz[g];
+			const T X0 = sx[g], Y0 = sy[g], Z0 = sz[g];
 			for (int t = 0; t < q; t += 4) {
-				const uint4 X4 = *reinterpret_cast<const uint4*>(bx + t);
-				const uint4 Y4 = *reinterpret_cast<const uint4*>(by + t);
-				const uint4 Z4 = *reinterpret_cast<const uint4*>(bz + t);
+				struct {
+					T x, y, z, w;
+				} X4, Y4, Z4;
+				load4(bx + t, X4.x, X4.y, X4.z, X4.w);
+				load4(by + t, Y4.x, Y4.y, Y4.z, Y4.w);
+				load4(bz + t, Z4.x, Z4.y, Z4.z, Z4.w);
 				if constexpr (near) {
 					pp_pair(X0, Y0, Z0, X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
 					pp_pair(X0, Y0, Z0, X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
@@ -1047,7 +1058,6 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	// sink leaf's: dist(centres) - r_s - r_o <= h (plus 1e-6 for the float
 	// rounding of the centres, radii and distance).  Pass 0 stages the far
 	// leaves, pass 1 the near ones.
-	const uint32_t cs0 = uniform(int(nodes[self].pos[0])), cs1 = uniform(int(nodes[self].pos[1])), cs2 = uniform(int(nodes[self].pos[2]));
 	const float rs = nodes[self].radius;
 	const int hpp = uniform(B->head[int64_t(kPP) * B->nnodes + self]);
 	// stage particles [pb, pe) into the tile, flushing whenever it fills
@@ -1058,9 +1068,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			for (int t = lane; t < m; t += 64) {
 				const int i = tn + t;
 				const int at = (i % P) * STR + i / P;
-				tx[at] = px[pb + t];
-				ty[at] = py[pb + t];
-				tz[at] = pz[pb + t];
+				tx[at] = T(int32_t(px[pb + t] - cs0));
+				ty[at] = T(int32_t(py[pb + t] - cs1));
+				tz[at] = T(int32_t(pz[pb + t] - cs2));
 			}
 			tn += m;
 			pb += m;
@@ -1370,7 +1380,7 @@ __global__ __launch_bounds__(256) void k_drift_batch(BoxDesc* __restrict__ boxes
 	const int bx = find_box(off, nbox, t);
 	const BoxDesc& B = boxes[bx];
 	const int i = t - off[bx];
-	if (B.rung[i] != B.drift_rung) return;
+	if (i < B.drift_first || B.rung[i] != B.drift_rung) return;
 	uint32_t* X[3] = {B.x, B.y, B.z};
 	float* V[3] = {B.vx, B.vy, B.vz};
 #pragma unroll
@@ -1933,9 +1943,10 @@ void launch_interactions(KickCtx* c, hipStream_t st, bool events) {
 	ca.out_cc = c->out[kCC].p;
 	ca.out_cp = c->out[kCP].p;
 	ca.cnt = c->cnt.p;
-	const int ng = std::max(1, std::min(c->total_nodes, 1 << 15));
+	const int ng = std::max(1, std::min(c->total_nodes, 1 << 24));
 	hipLaunchKernelGGL((k_cells_ewald<T>), dim3(ng), dim3(64), 0, st, ca);
-	hipLaunchKernelGGL((k_cells_direct<T>), dim3(ng), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL((k_cells_direct<T, false>), dim3(ng), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL((k_cells_direct<T, true>), dim3(ng), dim3(64), 0, st, ca);
 	if (events) (void) hipEventRecord(c->ev[2], st);
 	for (int d = 1; d <= c->max_depth; d++) {
 		const int tot = c->lvl_total[d];
@@ -2240,6 +2251,7 @@ int ct_tree_build_ex(ct_box_t b, double theta, int bucket, double hsoft, int whe
 	prm.theta = theta;
 	prm.bucket = bucket;
 	prm.hsoft = hsoft;
+	prm.first = b->rung_begin;
 	std::string err;
 	HostTree T;
 	if ((rc = build_tree(b->hp, prm, T, &err))) return fail(rc, err);
@@ -2352,6 +2364,7 @@ int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, con
 		d.drag_c = 1.0 / (1.0 + dg * double(dt[b]));
 		d.do_drag = dg != 0.0;
 		d.drift_rung = rung;
+		d.drift_first = int32_t(c->boxes[b]->rung_begin);
 	}
 	hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : c->own;
 	CT_HIP(hipMemcpyAsync(c->desc.p, c->hdesc.data(), c->hdesc.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, st));
@@ -2375,12 +2388,14 @@ int ct_drift(ct_box_t b, int rung, float a, float dt, double drag, void* stream)
 	if (rc) return rc;
 	if (!b || !(a > 0.0f)) return fail(CT_E_ARG, "ct_drift: bad arguments");
 	hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-	const int64_t n = b->hp.n;
-	if (n == 0) return CT_OK;
+	// the active range [rung_begin, n) (cuda_drift, src/drift.cu:170-177)
+	const int64_t f = b->rung_begin;
+	const int64_t n = b->hp.n - f;
+	if (n <= 0) return CT_OK;
 	const double c1 = double(dt) / double(a);
 	const double dragc = 1.0 / (1.0 + drag * double(dt));
-	hipLaunchKernelGGL(k_drift, dim3((n + 255) / 256), dim3(256), 0, st, b->x.p, b->y.p, b->z.p, b->vx.p, b->vy.p, b->vz.p, b->rung.p, n, rung,
-			c1, dragc, drag != 0.0);
+	hipLaunchKernelGGL(k_drift, dim3((n + 255) / 256), dim3(256), 0, st, b->x.p + f, b->y.p + f, b->z.p + f, b->vx.p + f, b->vy.p + f,
+			b->vz.p + f, b->rung.p + f, n, rung, c1, dragc, drag != 0.0);
 	CT_HIP(hipGetLastError());
 	CT_HIP(hipStreamSynchronize(st));
 	b->host_stale = true;

@@ -329,11 +329,11 @@ Built build(Ctx& c, int64_t b, int64_t e, const double gb[3], const double ge[3]
 }  // namespace
 
 int build_tree(HostParticles& p, const TreeBuildParams& prm, HostTree& out, std::string* err) {
-	if (p.n < 0 || p.n > INT32_MAX || prm.bucket < 1 || !(prm.theta > 0.0)) {
+	if (p.n < 0 || p.n > INT32_MAX || prm.bucket < 1 || !(prm.theta > 0.0) || prm.first < 0 || prm.first > p.n) {
 		if (err) *err = "build_tree: bad arguments";
 		return -1;
 	}
-	int64_t cap = 8 * p.n / prm.bucket + 65536;
+	int64_t cap = 8 * (p.n - prm.first) / prm.bucket + 65536;
 	for (int attempt = 0; attempt < 6; attempt++) {
 		std::vector<HostNode> nodes(cap);
 		Ctx c;
@@ -346,7 +346,7 @@ int build_tree(HostParticles& p, const TreeBuildParams& prm, HostTree& out, std:
 		c.fork_min = 1 << 16;
 		const double gb[3] = {0, 0, 0}, ge[3] = {1, 1, 1};
 		// restore the original order is not needed: a retry re-partitions
-		Built root = build(c, 0, p.n, gb, ge, 0);
+		Built root = build(c, prm.first, p.n, gb, ge, 0);
 		if (c.err == 2) {
 			cap *= 2;
 			continue;

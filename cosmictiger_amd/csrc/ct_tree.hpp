@@ -35,6 +35,7 @@ struct TreeBuildParams {
 	int bucket = 128;
 	double hsoft = 0.0;
 	int nthreads = 0;  // 0: hardware concurrency
+	int64_t first = 0; // the tree covers particles [first, n) (the active range)
 };
 
 struct HostTree {
@@ -44,8 +45,9 @@ struct HostTree {
 	int64_t nleaves = 0;
 };
 
-// Builds the tree over all particles, permuting them in place (the partition
-// of src/particles.cpp:979-1018).  Returns 0 or a negative error code.
+// Builds the tree over particles [prm.first, n) (particles_current_range,
+// src/particles.cpp:1020-1025), permuting them in place (the partition of
+// src/particles.cpp:979-1018).  Returns 0 or a negative error code.
 int build_tree(HostParticles& p, const TreeBuildParams& prm, HostTree& out, std::string* err);
 
 }  // namespace ct

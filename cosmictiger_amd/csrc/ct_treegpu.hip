@@ -76,12 +76,13 @@ struct GatherArgs {
 	int8_t* drung;
 	int32_t* did;
 	int n;
+	int first;  // particles below first keep their place
 };
 
 __global__ void k_gather(GatherArgs A) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= A.n) return;
-	const int j = A.idx[i];
+	const int j = i < A.first ? i : A.first + A.idx[i - A.first];
 	A.dx[i] = A.sx[j];
 	A.dy[i] = A.sy[j];
 	A.dz[i] = A.sz[j];
@@ -120,7 +121,7 @@ __global__ void k_split_flags(const TNode* __restrict__ nodes, int lb, int le, c
 // Children of the splitting nodes of one level, written at the next level in
 // breadth-first order (the scan of the split flags).
 __global__ void k_children(TNode* __restrict__ nodes, int lb, int le, const int32_t* __restrict__ flag, const int32_t* __restrict__ scan,
-		const uint64_t* __restrict__ key, int next) {
+		const uint64_t* __restrict__ key, int koff, int next) {
 	const int q = lb + blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= le) return;
 	if (!flag[q - lb]) {
@@ -134,7 +135,7 @@ __global__ void k_children(TNode* __restrict__ nodes, int lb, int le, const int3
 	int a = t.pbegin, b = t.pend;
 	while (a < b) {
 		const int m = a + ((b - a) >> 1);
-		if (key[m] < hi) {
+		if (key[m - koff] < hi) {
 			a = m + 1;
 		} else {
 			b = m;
@@ -331,7 +332,11 @@ __global__ __launch_bounds__(64) void k_internal_geom(const TNode* __restrict__ 
 				r2max = fmax_std(r2max, r2);
 			}
 			for (int o = 32; o > 0; o >>= 1) r2max = fmax_std(r2max, __shfl_xor(r2max, o));
-			radius1 = double(__fsqrt_rn(r2max));
+			// correctly rounded float square root, as the host's sqrtf: the
+			// double root rounded to float (double rounding cannot change a
+			// square root when 53 >= 2 * 24 + 2); the device's float root
+			// can be an ulp low
+			radius1 = double(float(__dsqrt_rn(double(r2max))));
 		}
 		const double Rl = L.radius, Rr = R.radius;
 		double N[3];
@@ -448,6 +453,9 @@ static std::vector<uint8_t> ewald_table(double theta, double hsoft) {
 
 int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeResult* res) {
 	const int n = int(b->hp.n);
+	// the tree covers the active range [rb, n) (particles_current_range)
+	const int rb = int(b->rung_begin);
+	const int m = n - rb;
 	hipStream_t st = nullptr;
 	int rc;
 	DBuf<uint64_t> kin, kout;
@@ -471,17 +479,18 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		tid.release();
 	};
 	const int64_t nn1 = std::max(n, 1);
-	if ((rc = kin.grow(nn1)) || (rc = kout.grow(nn1)) || (rc = iin.grow(nn1)) || (rc = iout.grow(nn1)) || (rc = tx.grow(nn1)) ||
+	const int64_t nm1 = std::max(m, 1);
+	if ((rc = kin.grow(nm1)) || (rc = kout.grow(nm1)) || (rc = iin.grow(nm1)) || (rc = iout.grow(nm1)) || (rc = tx.grow(nn1)) ||
 			(rc = ty.grow(nn1)) || (rc = tz.grow(nn1)) || (rc = tvx.grow(nn1)) || (rc = tvy.grow(nn1)) || (rc = tvz.grow(nn1)) ||
 			(rc = trung.grow(nn1)) || (rc = tid.grow(nn1))) {
 		cleanup();
 		return rc;
 	}
 	// 1. sort by Morton key, gather the particle arrays
-	if (n > 0) {
-		hipLaunchKernelGGL(k_keys, dim3((n + 255) / 256), dim3(256), 0, st, b->x.p, b->y.p, b->z.p, n, kin.p, iin.p);
+	if (m > 0) {
+		hipLaunchKernelGGL(k_keys, dim3((m + 255) / 256), dim3(256), 0, st, b->x.p + rb, b->y.p + rb, b->z.p + rb, m, kin.p, iin.p);
 		size_t tb = 0;
-		if (rocprim::radix_sort_pairs(nullptr, tb, kin.p, kout.p, iin.p, iout.p, size_t(n), 0, kKeyBits, st) != hipSuccess) {
+		if (rocprim::radix_sort_pairs(nullptr, tb, kin.p, kout.p, iin.p, iout.p, size_t(m), 0, kKeyBits, st) != hipSuccess) {
 			cleanup();
 			return fail(CT_E_HIP, "gpu tree: radix sort size query failed");
 		}
@@ -490,7 +499,7 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 			cleanup();
 			return rc;
 		}
-		hipError_t e = rocprim::radix_sort_pairs(tmp.p, tb, kin.p, kout.p, iin.p, iout.p, size_t(n), 0, kKeyBits, st);
+		hipError_t e = rocprim::radix_sort_pairs(tmp.p, tb, kin.p, kout.p, iin.p, iout.p, size_t(m), 0, kKeyBits, st);
 		(void) hipStreamSynchronize(st);
 		tmp.release();
 		if (e != hipSuccess) {
@@ -516,6 +525,7 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		ga.drung = trung.p;
 		ga.did = tid.p;
 		ga.n = n;
+		ga.first = rb;
 		hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, st, ga);
 		CT_HIP(hipGetLastError());
 		std::swap(b->x, tx);
@@ -543,7 +553,7 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 	{
 		TNode root;
 		root.prefix = 0;
-		root.pbegin = 0;
+		root.pbegin = rb;
 		root.pend = n;
 		root.depth = 0;
 		root.parent = -1;
@@ -596,7 +606,7 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 			bigger.p = nullptr;
 			cap = ncap;
 		}
-		hipLaunchKernelGGL(k_children, dim3((cnt + 255) / 256), dim3(256), 0, st, tn.p, lb, le, flag.p, scan.p, kout.p, le);
+		hipLaunchKernelGGL(k_children, dim3((cnt + 255) / 256), dim3(256), 0, st, tn.p, lb, le, flag.p, scan.p, kout.p, rb, le);
 		CT_HIP(hipGetLastError());
 		if (nsplit == 0) break;
 		level_off.push_back(next);

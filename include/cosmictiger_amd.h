@@ -166,11 +166,91 @@ int ct_batch_drift(ct_batch_t batch, int rung, const float* a, const float* dt, 
 /* Phase times and event counts of the batch's last kick, as ct_kick_stats. */
 int ct_batch_stats(ct_batch_t batch, double* out, int n);
 
-/* Drift particles of rung `rung` by dt at scale factor a (float arguments as
- * the reference narrows them), with optional drag.  Replaces
+/* Drift the particles of rung `rung` in the active range by dt at scale
+ * factor a (float arguments as the reference narrows them), with optional
+ * drag.  Replaces
  * cuda_drift(char, float a, float dt, ...) (cosmictiger/drift.hpp:54,
  * src/drift.cu:156) as called by drift() (src/drift.cpp:33). */
 int ct_drift(ct_box_t box, int rung, float a, float dt, double drag, void* stream);
+
+/* ---- active range and rungs ----
+ * A box keeps an active range [rung_begin, N): tree builds, kicks and drifts
+ * cover it (particles_current_range, src/particles.cpp:1020-1025).  It starts
+ * as the whole box. */
+
+/* Partition the active range so that particles with rung >= min_rung come
+ * last and make them the active range; min_rung = 0 makes the whole box
+ * active without moving anything.  *rung_begin (may be NULL) receives the new
+ * start.  The order inside each part is not the reference's (compare by id);
+ * the saved forces are not moved, as in the reference.  The box's tree is
+ * invalidated when particles move.  Replaces particles_sort_by_rung
+ * (src/particles.cpp:1029-1068). */
+int ct_box_sort_by_rung(ct_box_t box, int min_rung, int64_t* rung_begin);
+
+/* The active range [*first, *second).  Replaces particles_current_range
+ * (src/particles.cpp:1020-1025). */
+int ct_box_current_range(ct_box_t box, int64_t* first, int64_t* second);
+
+/* Save / restore the start of the active range.  Replace particles_push_rungs
+ * and particles_pop_rungs (src/particles.cpp:218-235); popping an empty stack
+ * is CT_E_STATE. */
+int ct_box_push_rungs(ct_box_t box);
+int ct_box_pop_rungs(ct_box_t box);
+
+/* rung = max(rung, min_rung) for every particle.  Replaces
+ * particles_set_minrung (src/particles.cpp:628-649). */
+int ct_box_set_minrung(ct_box_t box, int min_rung);
+
+/* counts[r] = number of particles on rung r for r < cap; *nrungs (may be
+ * NULL) receives 1 + the highest rung present (0 for an empty box).  Replaces
+ * particles_rung_counts (src/particles.cpp:583-624). */
+int ct_box_rung_counts(ct_box_t box, int64_t* counts, int cap, int* nrungs);
+
+/* One call of the hierarchical step (the kick ladder of
+ * kick_step_hierarchical, src/driver.cpp:382-593): ascending kicks from
+ * max_rung down to minrung, the top kick at minrung, then descending kicks
+ * and drifts of every rung the top kick and those after it raise, with the
+ * reference's tree builds over the active range, its rung sorts and its
+ * push/pop of the range, and its clipping of the top rung.  The expansion
+ * factor stays `scale` over the step (the static-box case; the reference's
+ * cosmological a(t) update, driver.cpp:561, belongs to its cosmology). */
+typedef struct ct_step_params {
+	int minrung;   /* min_rung(itime) raised to minrung0 (driver.cpp:990) */
+	int max_rung;  /* max_rung of the previous call */
+	int minrung0;  /* the run's lowest rung */
+	double scale;  /* a */
+	double tau;    /* time at the start of the call; first_call = (tau == 0) */
+	double t0;     /* top-level step */
+	double theta;
+	int bucket;
+	double hsoft;
+	double GM;
+	double eta;
+	double max_dt; /* SCALE_DT / adot (driver.cpp:501) */
+	double sign;   /* 1, or -1 for glass creation */
+	int do_phi;    /* full_eval */
+	int nocrop;
+	double drag;
+	int save_force;
+	int precision; /* CT_PREC_* */
+	int tree_where;/* CT_TREE_* */
+} ct_step_params;
+
+typedef struct ct_step_return {
+	ct_kick_return kr; /* the top kick's kick_return, max_rung = the new max rung */
+	int minrung;       /* minrung after the call (raised by one when the top was clipped) */
+	int nkicks;        /* kicks (tree builds) made */
+	int clip_top;
+	double tckin;      /* sum of dkin over the kicks (energies->tckin) */
+	double parts_processed;
+} ct_step_return;
+
+int ct_step_hierarchical(ct_box_t box, const ct_step_params* params, ct_step_return* out, void* stream);
+
+/* The integer time of the driver: inc() and min_rung() of
+ * cosmictiger/time.hpp:28-43. */
+uint64_t ct_time_inc(uint64_t t, int max_rung);
+int ct_time_min_rung(uint64_t t);
 
 /* Number of particles and nodes of a box. */
 int ct_box_size(ct_box_t box, int64_t* nparts, int64_t* nnodes);

@@ -336,3 +336,201 @@ def default_params(theta, h, GM, eta=None, do_phi=True):
     p.top = True
     p.do_phi = do_phi
     return p
+
+
+# ---------------------------------------------------------------------------
+# active range and the hierarchical step (src/particles.cpp:218-235,583-649,
+# 1020-1068; src/driver.cpp:382-593), restated over the oracle's tree build,
+# kick and drift.  Particles are kept in the order the steps leave them in;
+# tdrift[i] accumulates the drift time of each particle (moves with it).
+# ---------------------------------------------------------------------------
+
+RUNG_DT = [2.0 ** -r for r in range(64)]  # rung_dt (cosmictiger/particles.hpp:37)
+
+
+class OracleBox:
+    def __init__(self, x, y, z, vel, rung):
+        self.x = np.array(x, dtype=np.uint32)
+        self.y = np.array(y, dtype=np.uint32)
+        self.z = np.array(z, dtype=np.uint32)
+        self.n = len(self.x)
+        self.vel = np.array(vel, dtype=np.float32).reshape(self.n, 3)
+        self.rung = np.array(rung, dtype=np.int8)
+        self.id = np.arange(self.n, dtype=np.int32)
+        self.tdrift = np.zeros(self.n)
+        self.rung_begin = 0
+        self.stack = []
+        self.forces = None  # forces of the last kick, by particle id
+
+    def _permute_range(self, b, perm):
+        for name in ("x", "y", "z", "vel", "rung", "id", "tdrift"):
+            a = getattr(self, name)
+            a[b:] = a[b:][perm]
+
+    def sort_by_rung(self, min_rung):
+        """particles_sort_by_rung (src/particles.cpp:1029-1068): the active
+        range keeps the particles of rung >= min_rung, moved to its end."""
+        if min_rung == 0:
+            self.rung_begin = 0
+            return 0
+        b = self.rung_begin
+        low = self.rung[b:] < min_rung
+        perm = np.concatenate([np.nonzero(low)[0], np.nonzero(~low)[0]])
+        self._permute_range(b, perm)
+        self.rung_begin = b + int(np.count_nonzero(low))
+        return self.rung_begin
+
+    def push_rungs(self):
+        self.stack.append(self.rung_begin)
+
+    def pop_rungs(self):
+        self.rung_begin = self.stack.pop()
+
+    def set_minrung(self, m):
+        np.maximum(self.rung, np.int8(m), out=self.rung)
+
+    def rung_counts(self):
+        return np.bincount(self.rung.astype(np.int64), minlength=0) if self.n else np.zeros(0, np.int64)
+
+    def tree_kick(self, params, theta, bucket, h, nthreads=8):
+        """tree_create over the active range, then kick() of that tree
+        (src/driver.cpp:470-526)."""
+        b = self.rung_begin
+        T = OracleTree(self.x[b:], self.y[b:], self.z[b:], bucket, theta, h, nthreads=nthreads)
+        perm = T.id.astype(np.int64)
+        self._permute_range(b, perm)
+        vel = np.ascontiguousarray(self.vel[b:])
+        rung = np.ascontiguousarray(self.rung[b:])
+        forces, ret, _ = kick(T, params, vel, rung, want_forces=True, nthreads=nthreads)
+        self.vel[b:] = vel
+        self.rung[b:] = rung
+        self.forces = {"id": self.id[b:].copy(), **{k: v.copy() for k, v in forces.items()}}
+        return ret
+
+    def drift(self, level, a, dt):
+        """cuda_drift over the active range (src/drift.cu:156-195)."""
+        b = self.rung_begin
+        sel = self.rung[b:] == level
+        self.x[b:], self.y[b:], self.z[b:] = drift(self.x[b:], self.y[b:], self.z[b:], self.vel[b:], self.rung[b:], level, a, dt)
+        self.tdrift[b:][sel] += np.float32(dt)
+
+    def by_id(self):
+        o = np.argsort(self.id)
+        return {"x": self.x[o], "y": self.y[o], "z": self.z[o], "vel": self.vel[o], "rung": self.rung[o], "tdrift": self.tdrift[o]}
+
+
+def step_hierarchical(box, minrung, max_rung, minrung0, scale, tau, t0, theta, bucket, h, GM, eta, max_dt=1e30, sign=1.0,
+                      do_phi=True, nocrop=False, drag=0.0, nthreads=8, trace=None):
+    """kick_step_hierarchical (src/driver.cpp:382-593) over an OracleBox, with
+    the expansion factor held at `scale` for the drifts.  Returns
+    (kick_return of the top kick with max_rung set, minrung after the call,
+    number of kicks).  trace (a list) receives one tuple per kick:
+    (level, ascending, top, descending, rung_begin, drifted)."""
+    levels = list(range(max_rung, minrung, -1)) + [minrung]
+    ascending = True
+    top = False
+    clip_top = False
+    kr = None
+    nk = 0
+    li = 0
+    while li < len(levels):
+        level = levels[li]
+        if level == minrung:
+            ascending = False
+            top = True
+        else:
+            top = False
+        if not ascending and not top:
+            box.push_rungs()
+        if not ascending:
+            box.sort_by_rung(level)
+        if top and minrung == minrung0:
+            counts = box.rung_counts()
+            if len(counts) > minrung0 + 1:
+                fast = int(np.sum(counts[minrung0 + 1:]))
+                slow = int(counts[minrung0])
+                if 3 * fast > slow:
+                    clip_top = not nocrop
+        p = KickParams()
+        p.top = True if (clip_top and level == minrung0 + 1) else top
+        p.sign = sign
+        p.ascending = ascending or top
+        p.descending = False if (clip_top and top) else ((not ascending) or top)
+        p.vel_init = False
+        p.node_load = 0.0
+        p.gpu = True
+        p.min_level = 9
+        p.save_force = True
+        p.GM = GM
+        p.eta = eta
+        p.h = h
+        p.a = scale
+        p.first_call = tau == 0.0
+        p.min_rung = level
+        p.do_phi = do_phi and top
+        p.t0 = t0
+        p.max_dt = max_dt
+        p.theta = theta
+        rb = box.rung_begin
+        tkr = box.tree_kick(p, theta, bucket, h, nthreads=nthreads)
+        nk += 1
+        if clip_top and top:
+            box.set_minrung(minrung0 + 1)
+        if top:
+            kr = tkr
+        if clip_top and top:
+            levels.append(minrung0 + 1)
+        elif (not ascending) or top:
+            kr.max_rung = tkr.max_rung
+            if tkr.max_rung > level:
+                levels.append(level + 1)
+        drifted = ((not ascending) or top) and not (clip_top and top)
+        if drifted:
+            box.drift(level, scale, RUNG_DT[level] * t0)
+        if trace is not None:
+            trace.append((level, bool(ascending), bool(top), bool(p.descending), rb, bool(drifted)))
+        if ascending and not top:
+            box.pop_rungs()
+        li += 1
+    if clip_top:
+        minrung += 1
+    return kr, minrung, nk
+
+
+def run_steps(box, ncalls, t0, theta, bucket, h, GM, eta, max_dt=1e30, scale=1.0, nocrop=True, step=None, trace=None, cycles=1):
+    """The driver's sub-step loop (src/driver.cpp:907-1087): minrung from the
+    integer time, one hierarchical step, max_rung from its kick_return, tau
+    advanced by t0 / 2^max_rung, the time by inc().  step (default
+    step_hierarchical over an OracleBox) is called as step(minrung,
+    max_rung, minrung0, tau); it returns (kick_return, minrung after).
+    Stops after ncalls calls or when the time has returned to 0 `cycles`
+    times."""
+    itime = 0
+    tau = 0.0
+    max_rung = 0
+    minrung0 = 0
+    calls = []
+    for _ in range(ncalls):
+        m = 64
+        t = itime
+        while (t & 1) == 0 and m != 0:
+            m -= 1
+            t >>= 1
+        this_minrung = max(m, minrung0)
+        if step is None:
+            kr, om, _ = step_hierarchical(box, this_minrung, max_rung, minrung0, scale, tau, t0, theta, bucket, h, GM, eta,
+                                          max_dt=max_dt, nocrop=nocrop, trace=trace)
+        else:
+            kr, om = step(this_minrung, max_rung, minrung0, tau)
+        if om != this_minrung:
+            minrung0 += 1
+        max_rung = kr.max_rung
+        calls.append((this_minrung, max_rung))
+        tau += t0 / (1 << max_rung)
+        if max_rung > 0:
+            itime = (itime + (1 << (64 - max_rung))) & ((1 << 64) - 1)
+        if itime == 0:
+            cycles -= 1
+            if cycles == 0:
+                break
+    return calls, tau

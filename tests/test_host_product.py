@@ -3,6 +3,7 @@ oracle, tree build against the oracle's restatement (bit-exact on integer and
 fixed32 fields), and the exported C-ABI."""
 import ctypes
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -124,7 +125,7 @@ def test_tree_invariants(ct):
 
 def test_abi_exports_every_declared_symbol(ct):
     text = open(ct.HEADER_PATH).read()
-    names = sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ct_\w+)\s*\(", text, flags=re.M)))
+    names = sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(ct_\w+)\s*\(", text, flags=re.M)))
     assert len(names) >= 15
     L = ct.lib()
     for nm in names:
@@ -137,6 +138,50 @@ def test_abi_struct_layouts(ct):
     assert ct.KickParams.ascending.offset == 48
     assert ctypes.sizeof(ct.KickReturn) == 64
     assert ctypes.sizeof(ct.Node) == 216
+
+
+def test_abi_struct_layouts_match_the_header(ct, tmp_path):
+    """The ctypes mirrors have the C layout of the header's structs: a C
+    program compiled against the header prints sizeof and every offsetof."""
+    structs = {"ct_kick_params": ct.KickParams, "ct_kick_return": ct.KickReturn, "ct_step_params": ct.StepParams,
+               "ct_step_return": ct.StepReturn}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % ct.HEADER_PATH, "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c11", "-o", str(exe), str(src)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().split("\n") if l)
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
+
+
+def test_time_counter_closed_form(ct):
+    """inc / min_rung of the driver's integer time (cosmictiger/time.hpp:28-43):
+    over one top-level step at max_rung R the counter visits 2^R sub-steps,
+    sub-step k having min_rung = R - (trailing zeros of k), 0 at k = 0, and
+    the library's C versions agree with the Python ones."""
+    L = ct.lib()
+    for R in range(0, 7):
+        t = 0
+        seen = []
+        while True:
+            seen.append(ct.time_min_rung(t))
+            assert L.ct_time_min_rung(t) == seen[-1]
+            assert L.ct_time_inc(t, R) == ct.time_inc(t, R)
+            t = ct.time_inc(t, R)
+            if t == 0 or R == 0:
+                break
+        assert len(seen) == 2 ** R
+        for k, m in enumerate(seen):
+            tz = (k & -k).bit_length() - 1 if k else None
+            assert m == (0 if k == 0 else R - tz)
 
 
 def test_abi_errors_without_device(ct):

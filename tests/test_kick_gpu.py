@@ -15,6 +15,8 @@ Bars:
   * kick_return: FP64 sums within 1e-4 relative (forces enter in FP32);
   * drift: positions bit-exact against the oracle's drift of the same input.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -229,7 +231,17 @@ def test_device_tree_equals_host_tree(gpu, kind, n, theta, bucket):
     na, nb = a.nodes(), b.nodes()
     assert len(na) == len(nb)
     for f in ("box_begin", "box_end", "child", "parent", "pbegin", "pend", "pos", "leaf", "depth"):
-        assert np.array_equal(na[f], nb[f]), f
+        if not np.array_equal(na[f], nb[f]):
+            bad = np.nonzero(np.any((na[f] != nb[f]).reshape(len(na), -1), axis=1))[0]
+            for q in bad[:6]:
+                print("node %d depth %d leaf %d parts [%d,%d) child %s: %s host %s device %s radius %r %r" % (
+                    q, na["depth"][q], na["leaf"][q], na["pbegin"][q], na["pend"][q], na["child"][q], f, na[f][q], nb[f][q],
+                    na["radius"][q], nb["radius"][q]))
+            out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+            if os.path.isdir(out):
+                np.savez(os.path.join(out, "tree_mismatch_%s_%d.npz" % (kind, n)), host=na, device=nb, x=x, y=y, z=z,
+                         ida=a.read()["id"], idb=b.read()["id"])
+        assert np.array_equal(na[f], nb[f]), "%s differs at %d nodes" % (f, int(np.sum(np.any((na[f] != nb[f]).reshape(len(na), -1), axis=1))))
     assert np.array_equal(na["radius"].view(np.uint32), nb["radius"].view(np.uint32))
     scale = np.maximum(np.abs(na["multi"][:, :1]), 1e-30) * (1000.0 * np.maximum(na["radius"], 1e-9))[:, None] ** 6
     assert np.max(np.abs(na["multi"] - nb["multi"]) / np.maximum(scale, 1e-30)) <= 1e-6
