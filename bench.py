@@ -16,8 +16,10 @@ k_pp launches of one kick: 21 flops per source-sink pair, the reference's own
 weight, src/gravity.cu:343) from HIP events recorded on the kick's stream;
 `accuracy` is the rms and 99th-percentile relative force error of the kicked
 box against the FP64 direct Ewald sum on a seeded sample of sinks;
-`cpu_baseline` times the CPU oracle (a plain-C FP64 restatement of the same
-walk, multithreaded) on a bounded sample of the same box on this host.
+`cpu_baseline` times the reference's CPU SIMD kick path (kick() and the
+cpu_gravity_* kernels of src/kick.cpp and src/gravity.cpp, restated in AVX2
+FP32 with OpenMP threads) on a bounded sample of the same box on this host,
+and compares its forces on the sample with the GPU kick's.
 """
 import argparse
 import json
@@ -217,30 +219,45 @@ def copy_calibration(mib):
         hip.hipFree(b)
 
 
-def cpu_baseline(x, y, z, args, h):
-    """Oracle kick on a bounded sample of sinks of the same box (CPU, this host)."""
+def cpu_baseline(x, y, z, args, h, gpu_forces=None):
+    """The reference's CPU SIMD kick path (kick() + cpu_gravity_*, restated in
+    AVX2 FP32 under oracle/) on a bounded sample of sinks of the same box, on
+    this host's cores.  The sample grows until it is ~10 s of CPU work.  With
+    gpu_forces (forces by original particle index), the CPU kick's forces on
+    the sample are compared with the GPU kick's."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ct_oracle as oc
     nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    tb = time.perf_counter()
     tr = oc.OracleTree(x, y, z, args.bucket, args.theta, h, nthreads=nthreads)
+    tree_s = time.perf_counter() - tb
     n = len(x)
     vel = np.zeros((n, 3), np.float32)
     rung = np.zeros(n, np.int8)
     p = oc.default_params(args.theta, h, 1.0 / n)
     sample = int(args.cpu_sample)
     start = n // 2 - sample // 2
-    # grow the sample until the timing is at least ~10 s of CPU work or the cap is hit
     while True:
+        v = vel.copy()
+        r = rung.copy()
         t0 = time.perf_counter()
-        oc.kick(tr, p, vel, rung, want_forces=False, nthreads=nthreads, sample=(start, start + sample))
+        f, _ = oc.cpu_simd_kick(tr, p, v, r, nthreads=nthreads, sample=(start, start + sample))
         dt = time.perf_counter() - t0
         if dt * nthreads >= 10.0 or sample >= args.cpu_sample_max or sample >= n:
             break
         sample = min(n, sample * 4, args.cpu_sample_max)
         start = max(0, n // 2 - sample // 2)
-    return {"value": sample / dt, "unit": "particle-kicks/s", "cores": nthreads, "kind": "port",
-            "sample": "oracle kick (plain C FP64 restatement of the walk) of %d sinks [%d, %d) of the same %d-particle box, "
-                      "%.2f s wall on %d threads" % (sample, start, start + sample, n, dt, nthreads)}
+    out = {"value": sample / dt, "unit": "particle-kicks/s", "cores": nthreads, "kind": "port",
+           "sample": "CPU SIMD kick (the reference's CPU kick path, AVX2 FP32, GPU-path interaction lists) of %d sinks "
+                     "[%d, %d) of the same %d-particle box and tree, %.2f s wall on %d threads (tree build %.1f s, "
+                     "not timed)" % (sample, start, start + sample, n, dt, nthreads, tree_s)}
+    if gpu_forces is not None:
+        ids = tr.id[start:start + sample]
+        gc = np.stack([f["gx"][start:start + sample], f["gy"][start:start + sample], f["gz"][start:start + sample]], 1).astype(np.float64)
+        gg = np.stack([gpu_forces[k][ids] for k in ("gx", "gy", "gz")], 1).astype(np.float64)
+        e = np.linalg.norm(gc - gg, axis=1) / np.maximum(np.linalg.norm(gg, axis=1), 1e-300)
+        out["force_rel_diff_vs_gpu"] = {"rms": float(np.sqrt(np.mean(e ** 2))), "p99": float(np.percentile(e, 99))}
+    return out
 
 
 def main():
@@ -255,7 +272,7 @@ def main():
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4096)
-    ap.add_argument("--cpu-sample-max", type=int, default=262144)
+    ap.add_argument("--cpu-sample-max", type=int, default=4194304)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--accuracy-sinks", type=int, default=10000)
     ap.add_argument("--theta-accurate", type=float, default=0.4,
@@ -332,6 +349,13 @@ def main():
                 traffic = tr.get("bytes_per_launch")
         except Exception:
             traffic = None
+    gpu_forces = None
+    if not args.no_cpu_baseline:
+        # the timed kicks' forces, by original index, for the CPU baseline's check
+        d = box.read()
+        o = np.argsort(d["id"])
+        gpu_forces = {k: d[k][o] for k in ("gx", "gy", "gz")}
+        del d
     acc = accuracy(box, x, y, z, h, n, args.accuracy_sinks, args.seed + 1) if args.accuracy_sinks > 0 else None
     at_1e3 = None
     if args.theta_accurate > 0:
@@ -348,7 +372,7 @@ def main():
                   "accuracy": accuracy(box, x, y, z, h, n, args.accuracy_sinks, args.seed + 1) if args.accuracy_sinks > 0 else None}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x, y, z, args, h)
+        cpu = cpu_baseline(x, y, z, args, h, gpu_forces)
     # drift timing (reported beside the kick)
     td = time.perf_counter()
     box.drift(0, 1.0, 1.0 / 64)

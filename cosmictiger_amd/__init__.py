@@ -157,6 +157,8 @@ def _declare(L):
     L.ct_box_pop_rungs.argtypes = [_P]
     L.ct_box_set_minrung.argtypes = [_P, ctypes.c_int]
     L.ct_box_rung_counts.argtypes = [_P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.ct_box_save.argtypes = [_P, ctypes.c_char_p, ctypes.c_int]
+    L.ct_box_load.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(_P), _I64P]
     L.ct_step_hierarchical.argtypes = [_P, ctypes.POINTER(StepParams), ctypes.POINTER(StepReturn), _P]
     L.ct_time_inc.argtypes = [ctypes.c_uint64, ctypes.c_int]
     L.ct_time_inc.restype = ctypes.c_uint64
@@ -172,7 +174,7 @@ def _declare(L):
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
                  "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_kick", "ct_batch_drift",
                  "ct_batch_stats", "ct_box_sort_by_rung", "ct_box_current_range", "ct_box_push_rungs", "ct_box_pop_rungs",
-                 "ct_box_set_minrung", "ct_box_rung_counts", "ct_step_hierarchical"):
+                 "ct_box_set_minrung", "ct_box_rung_counts", "ct_step_hierarchical", "ct_box_save", "ct_box_load"):
         getattr(L, name).restype = ctypes.c_int
 
 
@@ -266,8 +268,12 @@ def tree_build_host(x, y, z, theta, bucket, hsoft):
 class Box:
     """A resident box: particles and tree in HBM, kicked and drifted there."""
 
-    def __init__(self, x, y, z, vel=None, rung=None, device=0):
+    def __init__(self, x, y, z, vel=None, rung=None, device=0, _handle=None):
         init(device)
+        if _handle is not None:
+            self._h = _handle
+            self.n = self.size()[0]
+            return
         self.n = len(x)
         x = np.ascontiguousarray(x, dtype=np.uint32)
         y = np.ascontiguousarray(y, dtype=np.uint32)
@@ -348,6 +354,19 @@ class Box:
         out = StepReturn()
         _check(lib().ct_step_hierarchical(self._h, ctypes.byref(params), ctypes.byref(out), stream))
         return out
+
+    def save(self, path, append=False):
+        """particles_save layout (src/particles.cpp:1206-1227)."""
+        _check(lib().ct_box_save(self._h, os.fsencode(path), int(bool(append))))
+
+    @classmethod
+    def load(cls, path, offset=0, device=0):
+        """particles_load (src/particles.cpp:1182-1204): (box, offset past the record)."""
+        init(device)
+        h = _P()
+        end = ctypes.c_int64()
+        _check(lib().ct_box_load(os.fsencode(path), int(offset), ctypes.byref(h), ctypes.byref(end)))
+        return cls(None, None, None, device=device, _handle=h), end.value
 
     def size(self):
         n = ctypes.c_int64()

@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 HOST_ARCH = ["-mavx2", "-mfma"]
-HIP_SOURCES = ["ct_kernels.hip", "ct_direct.hip", "ct_treegpu.hip", "ct_rungs.hip"]
+HIP_SOURCES = ["ct_kernels.hip", "ct_direct.hip", "ct_treegpu.hip", "ct_rungs.hip", "ct_io.hip"]
 
 
 def run(cmd):

@@ -273,7 +273,7 @@ def gen_greens():
     E.ind += 1
     E("X0 *= T(%r); X1 *= T(%r); X2 *= T(%r);" % (SCALE, SCALE, SCALE))
     E("const T r2 = ct_fma(X0, X0, ct_fma(X1, X1, X2 * X2));")
-    E("const T rinv = r2 > T(0) ? ct_rsqrt(r2) : T(0);")
+    E("const T rinv = ct_select(r2 > T(0), ct_rsqrt(r2), T(0));")
     E.flops += 3 + 5 + 2
     E("const T u_1_0_0 = X0 * rinv;")
     E("const T u_0_1_0 = X1 * rinv;")
@@ -566,10 +566,12 @@ def gen_m2m():
     return E
 
 
-def gen_ewald():
+def gen_ewald(masked=False):
     """Ewald correction D = d^n[f_periodic - f] at X, f = -1/r, f_periodic its
     zero-mean periodic sum (alpha = 2).  Inputs in box units; outputs in the
-    scaled frame like greens()."""
+    scaled frame like greens().  masked: the variant for short-vector types
+    (CPU SIMD), where the image cutoff is a lane mask: the block runs when any
+    lane is inside it and the radial functions of the other lanes are zero."""
     E = Emitter()
     P = ORDER
     outs = trless_set(P)
@@ -581,7 +583,7 @@ def gen_ewald():
     E("// s_l = (+1, -1, -1, +1)[l %% 4] multiplying (cos, sin, cos, sin) of 2 pi h.X.")
     E("// TC: storage type of the coefficient table (float for FP32 kicks, double for FP64).")
     E("template<class T, class TC = float>")
-    E("CT_HD inline int ewald_greens(T* __restrict__ D, T X0, T X1, T X2, const float* __restrict__ rimg, int nreal,")
+    E("CT_HD inline int ewald_greens%s(T* __restrict__ D, T X0, T X1, T X2, const float* __restrict__ rimg, int nreal," % ("_masked" if masked else ""))
     E("\t\tconst float* __restrict__ fh, const TC* __restrict__ fc, int nfour) {")
     E.ind += 1
     for n in outs:
@@ -638,12 +640,22 @@ def gen_ewald():
     E("const T d1 = X1 - T(rimg[3 * i + 1]);")
     E("const T d2 = X2 - T(rimg[3 * i + 2]);")
     E("const T r2 = ct_fma(d0, d0, ct_fma(d1, d1, d2 * d2));")
-    E("if (r2 < T(%r)) {" % (2.6 * 2.6))
-    E.ind += 1
-    E("const T r = ct_sqrt(r2);")
-    E("const T rinv2 = T(1) / r2;")
-    E("const T ex = ct_exp(T(%r) * r2);" % (-a * a))
-    E("const T b0 = ct_erfc(T(%r) * r) / r;" % a)
+    if masked:
+        E("const auto inside = r2 < T(%r);" % (2.6 * 2.6))
+        E("if (ct_any(inside)) {")
+        E.ind += 1
+        E("const T rr2 = ct_select(inside, r2, T(1));")
+        E("const T r = ct_sqrt(rr2);")
+        E("const T rinv2 = T(1) / rr2;")
+        E("const T ex = ct_select(inside, ct_exp(T(%r) * rr2), T(0));" % (-a * a))
+        E("const T b0 = ct_select(inside, ct_erfc(T(%r) * r) / r, T(0));" % a)
+    else:
+        E("if (r2 < T(%r)) {" % (2.6 * 2.6))
+        E.ind += 1
+        E("const T r = ct_sqrt(r2);")
+        E("const T rinv2 = T(1) / r2;")
+        E("const T ex = ct_exp(T(%r) * r2);" % (-a * a))
+        E("const T b0 = ct_erfc(T(%r) * r) / r;" % a)
     # B_k = ((2k-1) B_(k-1) + (2a^2)^k/(a sqrt(pi)) e) / r^2 ; g_k = (-1)^k B_k
     E("T e = T(%r) * ex;" % (1.0 / (a * math.sqrt(math.pi))))
     E("T b = b0;")
@@ -705,7 +717,7 @@ def main():
     parts = []
     gens = [("GREENS", gen_greens()), ("M2L", gen_m2l(ORDER, "M2L")), ("M2L2", gen_m2l(2, "M2L2")),
             ("L2L", gen_l2l(False)), ("L2P", gen_l2l(True)), ("P2M", gen_p2m()), ("M2M", gen_m2m()),
-            ("EWALD", gen_ewald())]
+            ("EWALD", gen_ewald()), ("EWALD_MASKED", gen_ewald(masked=True))]
     text = HEADER
     for name, E in gens:
         text += "#define CT_FLOPS_%s %d\n" % (name, E.flops)

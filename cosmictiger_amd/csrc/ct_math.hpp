@@ -13,6 +13,16 @@
 
 namespace ctfmm {
 
+// Lane select and "any lane" test; for scalar types a plain conditional.
+// Short-vector types (the CPU SIMD kick) provide their own overloads.
+template<class T>
+CT_HD inline T ct_select(bool c, T a, T b) {
+	return c ? a : b;
+}
+CT_HD inline bool ct_any(bool c) {
+	return c;
+}
+
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline float ct_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ inline double ct_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }

@@ -100,9 +100,11 @@ int ct_box_create(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_
 /* Free a box (particles_destroy / tree_destroy). */
 int ct_box_destroy(ct_box_t box);
 
-/* Build the tree over the box with opening angle theta, leaf bucket size and
- * softening hsoft; permutes the particles as the reference's partition does.
- * Replaces tree_create (cosmictiger/tree.hpp:207, src/tree.cpp:273). */
+/* Build the tree over the box's active range (all particles unless
+ * ct_box_sort_by_rung narrowed it) with opening angle theta, leaf bucket size
+ * and softening hsoft; permutes the particles of the range as the reference's
+ * partition does.  Replaces tree_create (cosmictiger/tree.hpp:207,
+ * src/tree.cpp:273, whose root covers particles_current_range(), :292). */
 int ct_tree_build(ct_box_t box, double theta, int bucket, double hsoft);
 
 /* Where ct_tree_build_ex builds the tree: AUTO = on the device (particles
@@ -251,6 +253,23 @@ int ct_step_hierarchical(ct_box_t box, const ct_step_params* params, ct_step_ret
  * cosmictiger/time.hpp:28-43. */
 uint64_t ct_time_inc(uint64_t t, int max_rung);
 int ct_time_min_rung(uint64_t t);
+
+/* ---- persistence ----
+ * A box in the reference's particle file layout (particles_save /
+ * particles_load, src/particles.cpp:1182-1227): int64 count; x, y, z
+ * (uint32 each); velocities (float[3] each); rungs (int8); int32 size of the
+ * active-range stack, int32 rung_begin, the stack (int32 each).  Particles are
+ * written in the box's current order; no tree is stored (a kick needs a tree
+ * build after a load, as after the reference's read_checkpoint). */
+
+/* Write the box to path (append != 0 appends, e.g. after a checkpoint header
+ * such as write_checkpoint's driver_params, src/driver.cpp:1161). */
+int ct_box_save(ct_box_t box, const char* path, int append);
+
+/* Create a box from the record at byte offset of path; *end_offset (may be
+ * NULL) receives the offset just past it.  Particle ids restart at 0 in file
+ * order. */
+int ct_box_load(const char* path, int64_t offset, ct_box_t* out, int64_t* end_offset);
 
 /* Number of particles and nodes of a box. */
 int ct_box_size(ct_box_t box, int64_t* nparts, int64_t* nnodes);

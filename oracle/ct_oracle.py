@@ -534,3 +534,44 @@ def run_steps(box, ncalls, t0, theta, bucket, h, GM, eta, max_dt=1e30, scale=1.0
             if cycles == 0:
                 break
     return calls, tau
+
+
+# ---------------------------------------------------------------------------
+# the CPU SIMD kick (oracle/ct_cpukick.cpp): the reference's CPU kick path
+# in AVX2 FP32, the timed CPU baseline
+# ---------------------------------------------------------------------------
+
+_CPK_PATH = os.path.join(_HERE, "_build", "libctcpukick.so")
+_cpk = None
+
+
+def cpk_lib():
+    global _cpk
+    if _cpk is None:
+        if not os.path.exists(_CPK_PATH):
+            build()
+        lib()  # oc_tree_build lives there
+        _cpk = ctypes.CDLL(_CPK_PATH)
+        _cpk.cpk_kick.argtypes = [ctypes.POINTER(Tree), ctypes.POINTER(KickParams), _U32, _U32, _U32, _F, _I8, _F, _F, _F, _F,
+                                  ctypes.POINTER(KickReturn), ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+        _cpk.cpk_kick.restype = ctypes.c_int
+    return _cpk
+
+
+def cpu_simd_kick(tree, params, vel, rung, nthreads=8, sample=None, cpu_rules=False):
+    """The CPU SIMD kick over an OracleTree (particles in tree order): vel
+    (n,3) float32 and rung int8 updated in place.  Returns (forces as float32
+    arrays, kick_return).  sample=(b, e) kicks only the particles [b, e).
+    cpu_rules: the CPU path's own PC / CP test instead of the GPU path's."""
+    n = tree.n
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    assert vel.shape == (n, 3) and rung.dtype == np.int8
+    g = [np.zeros(n, np.float32) for _ in range(4)]
+    ret = KickReturn()
+    sb, se = (0, n) if sample is None else sample
+    rc = cpk_lib().cpk_kick(ctypes.byref(tree.t), ctypes.byref(params), _p(tree.x, _U32), _p(tree.y, _U32), _p(tree.z, _U32), _p(vel, _F),
+                            _p(rung, _I8), *[_p(a, _F) for a in g], ctypes.byref(ret), int(nthreads), int(sb), int(se),
+                            int(bool(cpu_rules)))
+    if rc:
+        raise RuntimeError("CPU SIMD kick failed: %d" % rc)
+    return {"gx": g[0], "gy": g[1], "gz": g[2], "phi": g[3]}, ret
