@@ -191,6 +191,7 @@ def serving(ct, args):
         print("serve: rate %.2f/s batched %.3e p-steps/s p50 %.0f ms p99 %.0f ms | one box per launch %.3e p50 %.0f p99 %.0f"
               % (rate, b["throughput"], b["latency_ms"]["p50"], b["latency_ms"]["p99"], s1["throughput"], s1["latency_ms"]["p50"],
                  s1["latency_ms"]["p99"]), file=sys.stderr, flush=True)
+    ex.close()
     print(json.dumps({"metric": "serving: particle-steps/s and p50/p99 step latency (kick + drift + tree build per step)",
                       "unit": "particle-steps/s", "n_gpus": 1, "data": "synthetic",
                       "config": {"workload": "%d Poisson requests of 1-10 steps, N log-uniform in [16^3, 192^3], clustered U[0,0.8], "
@@ -268,7 +269,10 @@ def main():
     ap.add_argument("--n", type=int, default=256 ** 3)
     ap.add_argument("--workload", default="clustered", choices=["clustered", "uniform"])
     ap.add_argument("--theta", type=float, default=0.55)
-    ap.add_argument("--bucket", type=int, default=128)
+    ap.add_argument("--bucket", default="128",
+                    help="leaf bucket size, or 'auto': time a kick at each of --bucket-candidates on the resident box "
+                         "and keep the fastest")
+    ap.add_argument("--bucket-candidates", default="64,96,128,160,192")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=4096)
@@ -299,6 +303,10 @@ def main():
         sys.exit(2)
 
     import cosmictiger_amd as ct
+    if args.bucket != "auto":
+        args.bucket = int(args.bucket)
+    elif args.ensemble or args.serve:
+        args.bucket = 128  # the ensemble mode tunes its own bucket; serving uses the reference default
     if args.ensemble:
         ensemble(ct, args)
         return
@@ -311,10 +319,22 @@ def main():
     h = ct.softening(n)
     ct.init(0)
     box = ct.Box(x, y, z)
+    params = ct.default_kick_params(args.theta, h, 1.0 / n)
+    tuning = None
+    if args.bucket == "auto":
+        tuning = {}
+        for b in [int(v) for v in args.bucket_candidates.split(",")]:
+            box.tree_build(args.theta, b, h)
+            box.kick(params)
+            tk = time.perf_counter()
+            for _ in range(2):
+                box.kick(params)
+            tuning[b] = 1e3 * (time.perf_counter() - tk) / 2
+            print("bench: bucket %d: %.2f ms per kick" % (b, tuning[b]), file=sys.stderr, flush=True)
+        args.bucket = min(tuning, key=tuning.get)
     tb0 = time.perf_counter()
     box.tree_build(args.theta, args.bucket, h)
     tree_s = time.perf_counter() - tb0
-    params = ct.default_kick_params(args.theta, h, 1.0 / n)
     print("bench: %d particles, tree %.2fs, setup %.2fs" % (n, tree_s, time.perf_counter() - t0), file=sys.stderr, flush=True)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
@@ -394,7 +414,8 @@ def main():
                                "order 7, Ewald, seed %d" % (args.workload, n, args.theta, args.bucket, args.seed)
                    if args.workload == "clustered" else "uniform box %d particles, theta %.2f, bucket %d, order 7, Ewald, seed %d"
                    % (n, args.theta, args.bucket, args.seed),
-                   "n": n, "theta": args.theta, "bucket": args.bucket, "parallelism": "single GPU"},
+                   "n": n, "theta": args.theta, "bucket": args.bucket, "parallelism": "single GPU",
+                   "bucket_autotune_ms": tuning},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_pp (P2P, all leaf-size classes of one kick), FP32 vector ALU (compute bound, no MFMA); "

@@ -148,6 +148,7 @@ def _declare(L):
     L.ct_direct_ewald.argtypes = [_P, ctypes.c_int64, _P, _P, _P, ctypes.c_double, _P, _P, _P, _P, _P]
     L.ct_batch_create.argtypes = [_P, ctypes.c_int, ctypes.POINTER(_P)]
     L.ct_batch_destroy.argtypes = [_P]
+    L.ct_batch_reset.argtypes = [_P, _P, ctypes.c_int]
     L.ct_batch_kick.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P]
     L.ct_batch_drift.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
     L.ct_batch_stats.argtypes = [_P, _P, ctypes.c_int]
@@ -172,7 +173,7 @@ def _declare(L):
                  "ct_tree_build_host", "ct_kick",
                  "ct_kick_prec",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
-                 "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_kick", "ct_batch_drift",
+                 "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_reset", "ct_batch_kick", "ct_batch_drift",
                  "ct_batch_stats", "ct_box_sort_by_rung", "ct_box_current_range", "ct_box_push_rungs", "ct_box_pop_rungs",
                  "ct_box_set_minrung", "ct_box_rung_counts", "ct_step_hierarchical", "ct_box_save", "ct_box_load"):
         getattr(L, name).restype = ctypes.c_int
@@ -431,6 +432,12 @@ class Batch:
         h = _P()
         _check(lib().ct_batch_create(ctypes.cast(arr, _P), len(self.boxes), ctypes.byref(h)))
         self._h = h
+
+    def reset(self, boxes):
+        """Point the batch at another set of boxes, keeping its arenas."""
+        self.boxes = list(boxes)
+        arr = (_P * len(self.boxes))(*[b._h for b in self.boxes])
+        _check(lib().ct_batch_reset(self._h, ctypes.cast(arr, _P), len(self.boxes)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -97,6 +97,8 @@ struct WalkArgs {
 	Act* act[2];
 	int32_t* dp[2];
 	int32_t* ep[2];
+	uint8_t* dcat;  // per direct check of the current level: its class (pass 1 -> pass 2)
+	uint8_t* ecat;  // per Ewald check
 	int64_t cap_act, cap_dp, cap_ep;
 	int32_t* out[kNumLists];
 	int64_t cap_out[kNumLists];

@@ -92,4 +92,6 @@ struct ct_box {
 	// particles_push_rungs / particles_pop_rungs
 	int64_t rung_begin = 0;
 	std::vector<int64_t> rung_stack;
+	// device tree-build scratch kept between builds (ct_treegpu.hip)
+	void* tree_scratch = nullptr;
 };

@@ -187,8 +187,10 @@ struct WalkCounts {
 	int cc, cp, pc, lf, nx;
 };
 
+// Pass 1 of an entry: classifies every check, keeps each class for pass 2
+// (dcat / ecat at the check's place) and counts per list.
 __device__ __forceinline__ WalkCounts walk_count(const DNode* __restrict__ nodes, float thinv, float h, const DNode& s, const int32_t* dp,
-		const int32_t* ep, int db, int dn, int eb, int en, int lane) {
+		const int32_t* ep, uint8_t* __restrict__ dcat, uint8_t* __restrict__ ecat, int db, int dn, int eb, int en, int lane) {
 	WalkCounts w = {0, 0, 0, 0, 0, 0, 0, 0};
 	for (int base = 0; base < en; base += 64) {
 		const int i = base + lane;
@@ -196,6 +198,7 @@ __device__ __forceinline__ WalkCounts walk_count(const DNode* __restrict__ nodes
 		if (i < en) {
 			const DNode o = nodes[ep[eb + i]];
 			cat = classify_ewald(s, o, thinv);
+			ecat[eb + i] = uint8_t(cat);
 		}
 		w.ecc += __popcll(__ballot(cat == K_CC));
 		w.enext += __popcll(__ballot(cat == K_NEXT));
@@ -207,6 +210,7 @@ __device__ __forceinline__ WalkCounts walk_count(const DNode* __restrict__ nodes
 		if (i < dn) {
 			const DNode o = nodes[dp[db + i]];
 			cat = classify_direct(s, o, thinv, h);
+			dcat[db + i] = uint8_t(cat);
 		}
 		w.cc += __popcll(__ballot(cat == K_CC));
 		w.cp += __popcll(__ballot(cat == K_CP));
@@ -262,8 +266,8 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 			const DNode s = nodes[self];
 			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
 			const bool sleaf = s.flags & 1;
-			const WalkCounts w = walk_count(nodes, B->theta_inv, B->h, s, dp, ep, uniform(ae.db), uniform(ae.dn), uniform(ae.eb), uniform(ae.en),
-					lane);
+			const WalkCounts w = walk_count(nodes, B->theta_inv, B->h, s, dp, ep, A.dcat, A.ecat, uniform(ae.db), uniform(ae.dn), uniform(ae.eb),
+					uniform(ae.en), lane);
 			nchecks += uniform(ae.dn) + uniform(ae.en);
 			if (w.efail || (sleaf && w.enext)) {
 				if (lane == 0) atomicOr(&C->walk_error, 1);
@@ -359,11 +363,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 					const int i = bs + lane;
 					int cat = K_NONE;
 					int oi = 0;
-					DNode o;
 					if (i < en) {
 						oi = ep[eb + i];
-						o = nodes[oi];
-						cat = classify_ewald(s, o, thinv);
+						cat = A.ecat[eb + i];
 					}
 					const unsigned long long mcc = __ballot(cat == K_CC);
 					const unsigned long long mnx = __ballot(cat == K_NEXT);
@@ -371,11 +373,12 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 						A.out[kEwald][r_e + run_cc + prefix_below(mcc)] = oi;
 					}
 					if (cat == K_NEXT && ok[R_EP]) {
+						const int2 ch = *reinterpret_cast<const int2*>(nodes[oi].child);
 						const int pos = 2 * (run_nx + prefix_below(mnx));
-						nep[r_ep + pos] = o.child[0];
-						nep[r_ep + pos + 1] = o.child[1];
-						nep[r_ep + cen + pos] = o.child[0];
-						nep[r_ep + cen + pos + 1] = o.child[1];
+						nep[r_ep + pos] = ch.x;
+						nep[r_ep + pos + 1] = ch.y;
+						nep[r_ep + cen + pos] = ch.x;
+						nep[r_ep + cen + pos + 1] = ch.y;
 					}
 					run_cc += __popcll(mcc);
 					run_nx += __popcll(mnx);
@@ -388,11 +391,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 					const int i = bs + lane;
 					int cat = K_NONE;
 					int oi = 0;
-					DNode o;
 					if (i < dn) {
 						oi = dp[db + i];
-						o = nodes[oi];
-						cat = classify_direct(s, o, thinv, hs);
+						cat = A.dcat[db + i];
 					}
 					const unsigned long long mcc = __ballot(cat == K_CC);
 					const unsigned long long mcp = __ballot(cat == K_CP);
@@ -415,12 +416,13 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
 							ndp[r_dp + cdn + pos] = oi;
 						}
 					} else if (cat == K_NEXT && ok[R_DP]) {
+						const int2 ch = *reinterpret_cast<const int2*>(nodes[oi].child);
 						const int pos = 2 * (run_nx + prefix_below(mnx));
-						ndp[r_dp + pos] = o.child[0];
-						ndp[r_dp + pos + 1] = o.child[1];
+						ndp[r_dp + pos] = ch.x;
+						ndp[r_dp + pos + 1] = ch.y;
 						if (!sleaf) {
-							ndp[r_dp + cdn + pos] = o.child[0];
-							ndp[r_dp + cdn + pos + 1] = o.child[1];
+							ndp[r_dp + cdn + pos] = ch.x;
+							ndp[r_dp + cdn + pos + 1] = ch.y;
 						}
 					}
 					run_cc += __popcll(mcc);
@@ -1579,6 +1581,7 @@ struct KickCtx {
 	// walk arenas
 	DBuf<Act> act[2];
 	DBuf<int32_t> dp[2], ep[2];
+	DBuf<uint8_t> dcat, ecat;  // check classes of the level being walked
 	DBuf<int32_t> out[kNumLists];
 	DBuf<Seg> seg;
 	DBuf<int32_t> head;
@@ -1621,6 +1624,8 @@ struct KickCtx {
 			dp[q].release();
 			ep[q].release();
 		}
+		dcat.release();
+		ecat.release();
 		for (int q = 0; q < kNumLists; q++) out[q].release();
 		seg.release();
 		head.release();
@@ -1761,6 +1766,7 @@ int ctx_grow_arenas(KickCtx* c) {
 		if ((rc = c->dp[q].grow(c->cap_dp))) return rc;
 		if ((rc = c->ep[q].grow(c->cap_ep))) return rc;
 	}
+	if ((rc = c->dcat.grow(c->cap_dp)) || (rc = c->ecat.grow(c->cap_ep))) return rc;
 	for (int q = 0; q < kNumLists; q++) {
 		if ((rc = c->out[q].grow(c->cap_out[q]))) return rc;
 	}
@@ -1793,6 +1799,8 @@ WalkArgs walk_args(KickCtx* c) {
 		A.dp[q] = c->dp[q].p;
 		A.ep[q] = c->ep[q].p;
 	}
+	A.dcat = c->dcat.p;
+	A.ecat = c->ecat.p;
 	A.cap_act = c->cap_act;
 	A.cap_dp = c->cap_dp;
 	A.cap_ep = c->cap_ep;
@@ -1806,7 +1814,7 @@ WalkArgs walk_args(KickCtx* c) {
 	return A;
 }
 
-constexpr int kWalkGrid = 1024;  // 4096 waves, all resident at the walk's register use
+constexpr int kWalkGrid = 1280;  // 5120 waves: 5 per SIMD, all resident at the walk's register use (<= 102 VGPRs)
 
 int walk_iters(KickCtx* c) {
 	return c->max_depth + 3;
@@ -2139,6 +2147,7 @@ int ct_box_destroy(ct_box_t b) {
 	b->multi64.release();
 	b->L64.release();
 	b->pp.release();
+	tree_scratch_free(b);
 	delete b;
 	return CT_OK;
 }
@@ -2269,6 +2278,24 @@ int ct_batch_create(const ct_box_t* boxes, int nbox, ct_batch_t* out) {
 		return rc;
 	}
 	*out = bt;
+	return CT_OK;
+}
+
+int ct_batch_reset(ct_batch_t bt, const ct_box_t* boxes, int nbox) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!bt || nbox < 1 || !boxes) return fail(CT_E_ARG, "ct_batch_reset: bad arguments");
+	for (int i = 0; i < nbox; i++) {
+		if (!boxes[i]) return fail(CT_E_ARG, "ct_batch_reset: null box");
+		for (int j = 0; j < i; j++)
+			if (boxes[j] == boxes[i]) return fail(CT_E_ARG, "ct_batch_reset: a box appears twice");
+	}
+	CT_HIP(hipStreamSynchronize(bt->ctx.own));
+	KickCtx* c = &bt->ctx;
+	c->boxes.assign(boxes, boxes + nbox);
+	c->versions.assign(size_t(nbox), ~uint64_t(0));
+	c->prepared = false;
+	c->drop_graph();
 	return CT_OK;
 }
 

@@ -451,20 +451,21 @@ static std::vector<uint8_t> ewald_table(double theta, double hsoft) {
 	return t;
 }
 
-int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeResult* res) {
-	const int n = int(b->hp.n);
-	// the tree covers the active range [rb, n) (particles_current_range)
-	const int rb = int(b->rung_begin);
-	const int m = n - rb;
-	hipStream_t st = nullptr;
-	int rc;
+// Scratch of a box's device tree builds, kept with the box (grow-only) so
+// that rebuilding a resident box's tree allocates nothing after the first
+// build.  The particle arrays are swapped with the box's on each build.
+struct TreeScratch {
 	DBuf<uint64_t> kin, kout;
 	DBuf<int32_t> iin, iout;
 	DBuf<uint32_t> tx, ty, tz;
 	DBuf<float> tvx, tvy, tvz;
 	DBuf<int8_t> trung;
 	DBuf<int32_t> tid;
-	auto cleanup = [&]() {
+	DBuf<uint8_t> ewd, tmp;
+	DBuf<int32_t> err, flag, scan;
+	DBuf<TNode> tn;
+	DBuf<GNode> geo;
+	void release() {
 		kin.release();
 		kout.release();
 		iin.release();
@@ -477,7 +478,51 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		tvz.release();
 		trung.release();
 		tid.release();
-	};
+		ewd.release();
+		tmp.release();
+		err.release();
+		flag.release();
+		scan.release();
+		tn.release();
+		geo.release();
+	}
+};
+
+void tree_scratch_free(ct_box* b) {
+	if (b->tree_scratch) {
+		TreeScratch* S = static_cast<TreeScratch*>(b->tree_scratch);
+		S->release();
+		delete S;
+		b->tree_scratch = nullptr;
+	}
+}
+
+int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeResult* res) {
+	if (!b->tree_scratch) {
+		b->tree_scratch = new (std::nothrow) TreeScratch();
+		if (!b->tree_scratch) return fail(CT_E_OOM, "gpu tree: host allocation failed");
+	}
+	TreeScratch& S = *static_cast<TreeScratch*>(b->tree_scratch);
+	const int n = int(b->hp.n);
+	// the tree covers the active range [rb, n) (particles_current_range)
+	const int rb = int(b->rung_begin);
+	const int m = n - rb;
+	hipStream_t st = nullptr;
+	int rc;
+	auto& kin = S.kin;
+	auto& kout = S.kout;
+	auto& iin = S.iin;
+	auto& iout = S.iout;
+	auto& tx = S.tx;
+	auto& ty = S.ty;
+	auto& tz = S.tz;
+	auto& tvx = S.tvx;
+	auto& tvy = S.tvy;
+	auto& tvz = S.tvz;
+	auto& trung = S.trung;
+	auto& tid = S.tid;
+	auto& tmp = S.tmp;
+	auto cleanup = []() {};  // the scratch stays with the box
 	const int64_t nn1 = std::max(n, 1);
 	const int64_t nm1 = std::max(m, 1);
 	if ((rc = kin.grow(nm1)) || (rc = kout.grow(nm1)) || (rc = iin.grow(nm1)) || (rc = iout.grow(nm1)) || (rc = tx.grow(nn1)) ||
@@ -494,14 +539,12 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 			cleanup();
 			return fail(CT_E_HIP, "gpu tree: radix sort size query failed");
 		}
-		DBuf<uint8_t> tmp;
 		if ((rc = tmp.grow(int64_t(tb) + 1))) {
 			cleanup();
 			return rc;
 		}
 		hipError_t e = rocprim::radix_sort_pairs(tmp.p, tb, kin.p, kout.p, iin.p, iout.p, size_t(m), 0, kKeyBits, st);
 		(void) hipStreamSynchronize(st);
-		tmp.release();
 		if (e != hipSuccess) {
 			cleanup();
 			return fail(CT_E_HIP, std::string("gpu tree: radix sort: ") + hipGetErrorString(e));
@@ -539,11 +582,14 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 	}
 	// 2. topology, level by level
 	const std::vector<uint8_t> ew = ewald_table(theta, hsoft);
-	DBuf<uint8_t> ewd;
-	DBuf<int32_t> err, flag, scan;
+	auto& ewd = S.ewd;
+	auto& err = S.err;
+	auto& flag = S.flag;
+	auto& scan = S.scan;
 	std::vector<int32_t> level_off = {0, 1};
 	int64_t cap = std::max<int64_t>(4 * int64_t(n) / std::max(1, bucket / 2) + 1024, 1024);
-	DBuf<TNode> tn;
+	auto& tn = S.tn;
+	cap = std::max<int64_t>(cap, tn.n);
 	if ((rc = ewd.grow(int64_t(ew.size()))) || (rc = err.grow(1)) || (rc = tn.grow(cap))) {
 		cleanup();
 		return rc;
@@ -571,7 +617,6 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		hipLaunchKernelGGL(k_split_flags, dim3((cnt + 255) / 256), dim3(256), 0, st, tn.p, lb, le, ewd.p, bucket, flag.p, err.p);
 		size_t tb = 0;
 		(void) rocprim::exclusive_scan(nullptr, tb, flag.p, scan.p, 0, size_t(cnt) + 0, rocprim::plus<int32_t>(), st);
-		DBuf<uint8_t> tmp;
 		if ((rc = tmp.grow(int64_t(tb) + 1))) {
 			cleanup();
 			return rc;
@@ -584,7 +629,6 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		CT_HIP(hipMemcpy(&last_scan, scan.p + cnt - 1, 4, hipMemcpyDeviceToHost));
 		CT_HIP(hipMemcpy(&last_flag, flag.p + cnt - 1, 4, hipMemcpyDeviceToHost));
 		CT_HIP(hipMemcpy(&errv, err.p, 4, hipMemcpyDeviceToHost));
-		tmp.release();
 		if (errv) {
 			cleanup();
 			return fail(CT_E_TREE, "gpu tree: a node deeper than 63 levels must split");
@@ -614,7 +658,7 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 	const int nn = level_off.back();
 	const int max_depth = int(level_off.size()) - 2;
 	// 3. geometry, bottom-up
-	DBuf<GNode> geo;
+	auto& geo = S.geo;
 	if ((rc = geo.grow(nn)) || (rc = b->nodes.grow(nn)) || (rc = b->multi.grow(int64_t(nn) * kMultiStride)) ||
 			(rc = b->multi64.grow(int64_t(nn) * kMultiStride)) || (rc = b->L.grow(int64_t(nn) * kLStride))) {
 		cleanup();
@@ -644,13 +688,6 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 			res->leaf_pend.push_back(ht[q].pend);
 		}
 	}
-	geo.release();
-	tn.release();
-	ewd.release();
-	err.release();
-	flag.release();
-	scan.release();
-	cleanup();
 	return CT_OK;
 }
 

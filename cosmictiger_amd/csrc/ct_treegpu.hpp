@@ -21,4 +21,7 @@ struct GpuTreeResult {
 // split (the host build handles those).
 int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeResult* res);
 
+// Frees the tree-build scratch a box keeps between builds.
+void tree_scratch_free(ct_box* b);
+
 }  // namespace ct

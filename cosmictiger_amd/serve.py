@@ -78,6 +78,7 @@ class GpuExecutor:
         self.bucket = bucket
         self.dt = dt
         self.graph = graph
+        self.batch = None  # one batch context for every step: its arenas persist
 
     def create(self, req):
         from . import workloads
@@ -103,12 +104,13 @@ class GpuExecutor:
     def step(self, states):
         """One kick + drift of every box in states, together."""
         ct = self.ct
-        batch = ct.Batch([s["box"] for s in states])
-        try:
-            batch.kick([self.params(s) for s in states], graph=self.graph)
-            batch.drift(0, [1.0] * len(states), [self.dt] * len(states))
-        finally:
-            batch.close()
+        boxes = [s["box"] for s in states]
+        if self.batch is None:
+            self.batch = ct.Batch(boxes)
+        else:
+            self.batch.reset(boxes)
+        self.batch.kick([self.params(s) for s in states], graph=self.graph)
+        self.batch.drift(0, [1.0] * len(states), [self.dt] * len(states))
         for s in states:
             s["step"] += 1
 
@@ -123,6 +125,11 @@ class GpuExecutor:
 
     def release(self, st):
         st["box"].close()
+
+    def close(self):
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
 
 
 class FakeExecutor:

@@ -155,6 +155,12 @@ typedef struct ct_batch* ct_batch_t;
 int ct_batch_create(const ct_box_t* boxes, int nbox, ct_batch_t* out);
 int ct_batch_destroy(ct_batch_t batch);
 
+/* Point an existing batch at another set of boxes, keeping its walk arenas,
+ * list buffers and stream (a serving loop steps a different set of resident
+ * boxes each time without reallocating).  The next kick rebuilds the batch's
+ * index spaces.  No counterpart in the reference. */
+int ct_batch_reset(ct_batch_t batch, const ct_box_t* boxes, int nbox);
+
 /* One kick of every box of the batch with params[i] for box i; rets[i]
  * receives box i's kick_return.  use_graph != 0 replays a captured HIP graph
  * of the whole kick (captured on the first call after the batch's trees or

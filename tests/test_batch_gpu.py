@@ -109,3 +109,27 @@ def test_batch_rebuild_and_errors(gpu):
     ref.close()
     for b in boxes:
         b.close()
+
+
+def test_batch_reset_equals_fresh_batch(gpu):
+    """A batch pointed at another set of boxes (ct_batch_reset, arenas kept)
+    kicks them exactly as a new batch of those boxes does."""
+    specs = _boxes(gpu)
+    first = [_make(gpu, sp, 64) for sp in specs[:2]]
+    second_a = [_make(gpu, sp, 64) for sp in specs[2:]]
+    second_b = [_make(gpu, sp, 64) for sp in specs[2:]]
+    bt = gpu.Batch([b for b, _ in first])
+    bt.kick([p for _, p in first])
+    bt.reset([b for b, _ in second_a])
+    ra = bt.kick([p for _, p in second_a])
+    bt.close()
+    fresh = gpu.Batch([b for b, _ in second_b])
+    rb = fresh.kick([p for _, p in second_b])
+    fresh.close()
+    for (ba, _), (bb, _), xa, xb in zip(second_a, second_b, ra, rb):
+        assert xa.as_dict() == xb.as_dict()
+        da, db = ba.read(), bb.read()
+        for f in FIELDS:
+            assert np.array_equal(da[f], db[f]), f
+    for b, _ in first + second_a + second_b:
+        b.close()

@@ -23,3 +23,4 @@ def test_results_same_alone_and_in_the_mix(gpu):
         s1.close()
         for f in ("x", "y", "z", "vel", "rung"):
             assert np.array_equal(alone.result[f], r.result[f]), (r.rid, f)
+    ex.close()
