@@ -149,6 +149,7 @@ def _declare(L):
     L.ct_batch_create.argtypes = [_P, ctypes.c_int, ctypes.POINTER(_P)]
     L.ct_batch_destroy.argtypes = [_P]
     L.ct_batch_reset.argtypes = [_P, _P, ctypes.c_int]
+    L.ct_batch_step.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P, _P, _P, ctypes.c_int, _P]
     L.ct_batch_kick.argtypes = [_P, _P, ctypes.c_int, _P, ctypes.c_int, _P]
     L.ct_batch_drift.argtypes = [_P, ctypes.c_int, _P, _P, _P, _P]
     L.ct_batch_stats.argtypes = [_P, _P, ctypes.c_int]
@@ -173,7 +174,7 @@ def _declare(L):
                  "ct_tree_build_host", "ct_kick",
                  "ct_kick_prec",
                  "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
-                 "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_reset", "ct_batch_kick", "ct_batch_drift",
+                 "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_reset", "ct_batch_step", "ct_batch_kick", "ct_batch_drift",
                  "ct_batch_stats", "ct_box_sort_by_rung", "ct_box_current_range", "ct_box_push_rungs", "ct_box_pop_rungs",
                  "ct_box_set_minrung", "ct_box_rung_counts", "ct_step_hierarchical", "ct_box_save", "ct_box_load"):
         getattr(L, name).restype = ctypes.c_int
@@ -456,6 +457,19 @@ class Batch:
         parr = (KickParams * n)(*params)
         rets = (KickReturn * n)()
         _check(lib().ct_batch_kick(self._h, ctypes.cast(parr, _P), int(precision), ctypes.cast(rets, _P), int(bool(graph)), stream))
+        return list(rets)
+
+    def step(self, params, rung, a, dt, drag=None, precision=PREC_F32C, graph=False, stream=None):
+        """Kick then drift every box (one captured graph with graph=True).
+        Returns the list of KickReturn."""
+        n = len(self.boxes)
+        parr = (KickParams * n)(*params)
+        rets = (KickReturn * n)()
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), (n,)))
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (n,)))
+        dg = None if drag is None else np.ascontiguousarray(np.broadcast_to(np.asarray(drag, np.float64), (n,)))
+        _check(lib().ct_batch_step(self._h, ctypes.cast(parr, _P), int(precision), ctypes.cast(rets, _P), int(rung), _ptr(a), _ptr(dt),
+                                   _ptr(dg), int(bool(graph)), stream))
         return list(rets)
 
     def drift(self, rung, a, dt, drag=None, stream=None):

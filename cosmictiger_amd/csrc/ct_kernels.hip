@@ -1157,6 +1157,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 	using T = typename Prec<PR>::T;
 	using Acc = typename Prec<PR>::A;
 	__shared__ Acc acc[4 * kMaxLeafSinks];
+	// a walk that overflowed its arenas left incomplete lists: the kick is
+	// redone eagerly, so this pass must not touch the particles
+	if (__builtin_amdgcn_readfirstlane(A.cnt->overflow | A.cnt->walk_error)) return;
 	const int li = blockIdx.x;
 	const int2 le = A.leaves[li];
 	const BoxDesc Bv = load_uniform(A.boxes + uniform(le.x));
@@ -1377,9 +1380,13 @@ __global__ __launch_bounds__(256) void k_drift(uint32_t* __restrict__ x, uint32_
 
 // The drift of every box of a batch in one launch: thread t drifts particle
 // t - off[b] of box b (same arithmetic as k_drift, per-box c1 and drag).
-__global__ __launch_bounds__(256) void k_drift_batch(BoxDesc* __restrict__ boxes, const int32_t* __restrict__ off, int nbox, int total) {
+// cnt (may be null): the kick's counters; a step whose walk overflowed its
+// arenas (and is redone eagerly) does not drift.
+__global__ __launch_bounds__(256) void k_drift_batch(BoxDesc* __restrict__ boxes, const int32_t* __restrict__ off, int nbox, int total,
+		const Counters* __restrict__ cnt) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= total) return;
+	if (cnt && (cnt->overflow | cnt->walk_error)) return;
 	const int bx = find_box(off, nbox, t);
 	const BoxDesc& B = boxes[bx];
 	const int i = t - off[bx];
@@ -1592,6 +1599,7 @@ struct KickCtx {
 	hipGraphExec_t gexec = nullptr;
 	hipStream_t own = nullptr;
 	int graph_prec = -1;
+	bool graph_drift = false;  // the captured graph ends with the batch drift
 	// diagnostics of the last kick
 	hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 	double stats[15] = {0};
@@ -1604,6 +1612,7 @@ struct KickCtx {
 		if (gexec) (void) hipGraphExecDestroy(gexec);
 		gexec = nullptr;
 		graph_prec = -1;
+		graph_drift = false;
 	}
 	void release() {
 		drop_graph();
@@ -2007,7 +2016,16 @@ int ctx_finish(KickCtx* c, hipStream_t st, ct_kick_return* rets, bool events) {
 // interactions.  Graph: the whole kick replayed from a captured graph; if the
 // walk overflowed in the replay, the kick is repeated eagerly and the graph
 // captured again on the next call.
-int ctx_kick(KickCtx* c, const ct_kick_params* p, int precision, ct_kick_return* rets, bool graph, hipStream_t st) {
+void launch_drift_batch(KickCtx* c, hipStream_t st, bool guarded) {
+	if (c->total_parts > 0)
+		hipLaunchKernelGGL(k_drift_batch, dim3((c->total_parts + 255) / 256), dim3(256), 0, st, c->desc.p, c->part_off.p, c->nbox(),
+				c->total_parts, guarded ? c->cnt.p : nullptr);
+}
+
+// One kick of every box of the context; with drift, the batch drift of the
+// box table's drift fields follows the kick in the same stream (and in the
+// captured graph).
+int ctx_kick(KickCtx* c, const ct_kick_params* p, int precision, ct_kick_return* rets, bool graph, hipStream_t st, bool drift = false) {
 	int rc;
 	if (precision != CT_PREC_F32C && precision != CT_PREC_F32U && precision != CT_PREC_F64) return fail(CT_E_ARG, "kick: bad precision");
 	if ((rc = ctx_prepare(c))) return rc;
@@ -2015,24 +2033,27 @@ int ctx_kick(KickCtx* c, const ct_kick_params* p, int precision, ct_kick_return*
 	if (walk_iters(c) > kMaxIter) return fail(CT_E_TREE, "tree too deep for the walk");
 	if (graph && st == nullptr) st = c->own;
 	CT_HIP(hipMemcpyAsync(c->desc.p, c->hdesc.data(), c->hdesc.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, st));
-	if (graph && c->gexec && c->graph_prec == precision) {
+	if (graph && c->gexec && (c->graph_prec != precision || c->graph_drift != drift)) c->drop_graph();
+	if (graph && c->gexec) {
 		CT_HIP(hipEventRecord(c->ev[0], st));
 		CT_HIP(hipGraphLaunch(c->gexec, st));
 		if ((rc = ctx_finish(c, st, rets, false))) return rc;
 		if (!c->last_counters.overflow) return CT_OK;
-		c->drop_graph();  // overflowed: fall through to an eager kick
+		c->drop_graph();  // overflowed (the drift in the graph did not run): an eager step
 	}
 	CT_HIP(hipEventRecord(c->ev[0], st));
 	if ((rc = run_walk(c, st))) return rc;
 	CT_HIP(hipEventRecord(c->ev[1], st));
 	launch_interactions_prec(c, st, precision, true);
+	if (drift) launch_drift_batch(c, st, false);
 	if ((rc = ctx_finish(c, st, rets, true))) return rc;
 	if (graph && !c->gexec) {
-		// capture walk + interactions for the next calls (arenas now fit)
+		// capture walk + interactions (+ drift) for the next calls (arenas now fit)
 		hipGraph_t g = nullptr;
 		CT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
 		launch_walk(c, st);
 		launch_interactions_prec(c, st, precision, false);
+		if (drift) launch_drift_batch(c, st, true);
 		hipError_t e = hipStreamEndCapture(st, &g);
 		if (e != hipSuccess) return fail(CT_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
 		e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
@@ -2042,6 +2063,7 @@ int ctx_kick(KickCtx* c, const ct_kick_params* p, int precision, ct_kick_return*
 			return fail(CT_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
 		}
 		c->graph_prec = precision;
+		c->graph_drift = drift;
 	}
 	return CT_OK;
 }
@@ -2315,14 +2337,11 @@ int ct_batch_kick(ct_batch_t bt, const ct_kick_params* params, int precision, ct
 	return ctx_kick(&bt->ctx, params, precision, rets, use_graph != 0, st);
 }
 
-int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, const double* drag, void* stream) {
-	int rc = check_init();
-	if (rc) return rc;
-	if (!bt || !a || !dt) return fail(CT_E_ARG, "ct_batch_drift: bad arguments");
-	KickCtx* c = &bt->ctx;
-	if ((rc = ctx_prepare(c))) return rc;
+// The drift fields of the box table (cuda_drift's c1 = dt / a and drag
+// factor, src/drift.cu:54-73, per box).
+static int set_drift_fields(KickCtx* c, int rung, const float* a, const float* dt, const double* drag) {
 	for (int b = 0; b < c->nbox(); b++) {
-		if (!(a[b] > 0.0f)) return fail(CT_E_ARG, "ct_batch_drift: bad scale factor");
+		if (!(a[b] > 0.0f)) return fail(CT_E_ARG, "batch drift: bad scale factor");
 		BoxDesc& d = c->hdesc[b];
 		const double dg = drag ? drag[b] : 0.0;
 		d.drift_c1 = double(dt[b]) / double(a[b]);
@@ -2331,11 +2350,31 @@ int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, con
 		d.drift_rung = rung;
 		d.drift_first = int32_t(c->boxes[b]->rung_begin);
 	}
+	return CT_OK;
+}
+
+int ct_batch_step(ct_batch_t bt, const ct_kick_params* params, int precision, ct_kick_return* rets, int rung, const float* a, const float* dt,
+		const double* drag, int use_graph, void* stream) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!bt || !params || !a || !dt) return fail(CT_E_ARG, "ct_batch_step: bad arguments");
+	KickCtx* c = &bt->ctx;
+	if ((rc = ctx_prepare(c))) return rc;
+	if ((rc = set_drift_fields(c, rung, a, dt, drag))) return rc;
+	hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : c->own;
+	return ctx_kick(c, params, precision, rets, use_graph != 0, st, true);
+}
+
+int ct_batch_drift(ct_batch_t bt, int rung, const float* a, const float* dt, const double* drag, void* stream) {
+	int rc = check_init();
+	if (rc) return rc;
+	if (!bt || !a || !dt) return fail(CT_E_ARG, "ct_batch_drift: bad arguments");
+	KickCtx* c = &bt->ctx;
+	if ((rc = ctx_prepare(c))) return rc;
+	if ((rc = set_drift_fields(c, rung, a, dt, drag))) return rc;
 	hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : c->own;
 	CT_HIP(hipMemcpyAsync(c->desc.p, c->hdesc.data(), c->hdesc.size() * sizeof(BoxDesc), hipMemcpyHostToDevice, st));
-	if (c->total_parts > 0)
-		hipLaunchKernelGGL(k_drift_batch, dim3((c->total_parts + 255) / 256), dim3(256), 0, st, c->desc.p, c->part_off.p, c->nbox(),
-				c->total_parts);
+	launch_drift_batch(c, st, false);
 	CT_HIP(hipGetLastError());
 	CT_HIP(hipStreamSynchronize(st));
 	for (ct_box* x : c->boxes) x->host_stale = true;

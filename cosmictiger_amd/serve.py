@@ -109,8 +109,7 @@ class GpuExecutor:
             self.batch = ct.Batch(boxes)
         else:
             self.batch.reset(boxes)
-        self.batch.kick([self.params(s) for s in states], graph=self.graph)
-        self.batch.drift(0, [1.0] * len(states), [self.dt] * len(states))
+        self.batch.step([self.params(s) for s in states], 0, [1.0] * len(states), [self.dt] * len(states), graph=self.graph)
         for s in states:
             s["step"] += 1
 

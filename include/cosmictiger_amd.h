@@ -167,6 +167,15 @@ int ct_batch_reset(ct_batch_t batch, const ct_box_t* boxes, int nbox);
  * arenas change; a replay whose walk overflowed its arenas is redone eagerly). */
 int ct_batch_kick(ct_batch_t batch, const ct_kick_params* params, int precision, ct_kick_return* rets, int use_graph, void* stream);
 
+/* A kick of every box of the batch followed by the batch drift (rung, a[i],
+ * dt[i], drag[i] as ct_batch_drift), in one stream; with use_graph != 0 the
+ * kick and the drift are captured in one HIP graph and replayed.  A replay
+ * whose walk overflowed its arenas moves no particle and is redone eagerly.
+ * The serving form of one step of every box (kick then drift,
+ * src/driver.cpp:528,562). */
+int ct_batch_step(ct_batch_t batch, const ct_kick_params* params, int precision, ct_kick_return* rets, int rung, const float* a,
+		const float* dt, const double* drag, int use_graph, void* stream);
+
 /* Drift of every box of the batch in one launch (a[i], dt[i], drag[i] for
  * box i; drag may be NULL), with cuda_drift's arithmetic (src/drift.cu:36-154). */
 int ct_batch_drift(ct_batch_t batch, int rung, const float* a, const float* dt, const double* drag, void* stream);
