@@ -3,9 +3,11 @@
 
 Workload (default): 256^3 = 16,777,216 particles, 70% in 2000 NFW halos
 (masses log-uniform over 3 decades, concentration 5-15, truncated at r200) and
-30% uniform, seed 42; theta = 0.55, leaf bucket 128, Ewald on, expansion
-order 7, softening (1/80)(693/256) N^(-1/3), GM = 1/N; first kick of the
-top level (kick_params of src/driver.cpp:481-508 with tau = 0).
+30% uniform, seed 42; theta = 0.55, Ewald on, expansion order 7, softening
+(1/80)(693/256) N^(-1/3), GM = 1/N; first kick of the top level (kick_params
+of src/driver.cpp:481-508 with tau = 0).  The leaf bucket is autotuned on the
+resident box (a timed kick at each of 64, 96, 128, 160, 192; the reference's
+default is 128) unless --bucket names one.
 
 A step is one kick of every particle of the resident box: the walk, all
 interactions (M2L, Ewald, P2L, P2P, M2P, L2P) and the kick update.  Inputs are
@@ -175,8 +177,11 @@ def serving(ct, args):
     def run(rate, batched):
         reqs = serve.poisson_requests(nreq, rate, seed=77)
         srv = serve.Server(ex, hbm_budget=budget, host_threads=min(16, os.cpu_count() or 1), batched=batched)
+        for k in ex.timing:
+            ex.timing[k] = 0
         res = srv.run(reqs)
         res["max_batch"] = srv.max_batch
+        res["executor_time"] = dict(ex.timing)
         srv.close()
         return res
 
@@ -197,6 +202,79 @@ def serving(ct, args):
                       "config": {"workload": "%d Poisson requests of 1-10 steps, N log-uniform in [16^3, 192^3], clustered U[0,0.8], "
                                              "theta %.2f, bucket %d, HBM budget %d GiB" % (nreq, args.theta, args.bucket, budget >> 30)},
                       "saturation": {"rate_per_s": rate_sat, "run": sat}, "rates": rows}), flush=True)
+
+
+def config1(ct, args):
+    """The plain method: 64^3 uniform particles (seed 42), one kick at theta
+    0.7, bucket 128, Ewald on; the GPU kick and the CPU SIMD kick (the
+    reference's CPU path) of the whole box, both checked against the FP64
+    direct Ewald sum on 4096 seeded sinks."""
+    from cosmictiger_amd import workloads
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ct_oracle as oc
+    n = 64 ** 3
+    theta, bucket = 0.7, 128
+    x, y, z = workloads.uniform_box(n, 42)
+    h = ct.softening(n)
+    ct.init(0)
+    box = ct.Box(x, y, z)
+    box.tree_build(theta, bucket, h)
+    p = ct.default_kick_params(theta, h, 1.0 / n)
+    box.kick(p)
+    steps = 10
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        box.kick(p)
+    gpu_s = (time.perf_counter() - t0) / steps
+    ref = direct_reference(box, x, y, z, h, n, 4096, 7)
+    acc_gpu = accuracy(box, x, y, z, h, n, 4096, 7, ref=ref)
+    nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    tr = oc.OracleTree(x, y, z, bucket, theta, h, nthreads=nthreads)
+    po = oc.default_params(theta, h, 1.0 / n)
+    t0 = time.perf_counter()
+    f, _ = oc.cpu_simd_kick(tr, po, np.zeros((n, 3), np.float32), np.zeros(n, np.int8), nthreads=nthreads)
+    cpu_s = time.perf_counter() - t0
+    inv = np.empty(n, np.int64)
+    inv[tr.id] = np.arange(n)
+    sinks, rf = ref
+    g = np.stack([f["gx"][inv][sinks], f["gy"][inv][sinks], f["gz"][inv][sinks]], 1).astype(np.float64)
+    gr = (1.0 / n) * np.stack([rf["gx"], rf["gy"], rf["gz"]], 1)
+    e = np.linalg.norm(g - gr, axis=1) / np.linalg.norm(gr, axis=1)
+    box.close()
+    print(json.dumps({"metric": "config 1 (plain method): particle-kicks/s and force error vs FP64 direct Ewald",
+                      "unit": "particle-kicks/s", "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": "uniform box 262144 particles, seed 42, theta 0.7, bucket 128, order 7, Ewald"},
+                      "gpu": {"value": n / gpu_s, "ms_per_kick": 1e3 * gpu_s, "accuracy": acc_gpu},
+                      "cpu_simd": {"value": n / cpu_s, "s_per_kick": cpu_s, "cores": nthreads,
+                                   "accuracy": {"sinks": int(len(sinks)), "rms": float(np.sqrt(np.mean(e ** 2))),
+                                                "p99": float(np.percentile(e, 99))}}}), flush=True)
+
+
+def drift_check(ct, box, args):
+    """The fixed32 drift of every particle (each rung present, dt = rung_dt
+    [rung] t0 with t0 = 1/64, a = 1; cuda_drift, src/drift.cu:36-154): device
+    time and HBM rate at 37 B per particle, and the positions compared bit for
+    bit with the oracle's drift of the same input."""
+    d0 = box.read()
+    rungs = sorted(set(np.unique(d0["rung"]).tolist()))
+    t0 = 1.0 / 64
+    ms = 0.0
+    for r in rungs:
+        td = time.perf_counter()
+        box.drift(r, 1.0, t0 * 2.0 ** -r)
+        ms += 1e3 * (time.perf_counter() - td)
+    d1 = box.read()
+    n = len(d0["x"])
+    out = {"rungs": rungs, "ms": ms, "bytes_per_particle": 37, "GB_per_s": 37.0 * n / (ms * 1e-3) / 1e9,
+           "hbm_frac": 37.0 * n / (ms * 1e-3) / 8e12}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ct_oracle as oc
+        x, y, z = d0["x"], d0["y"], d0["z"]
+        for r in rungs:
+            x, y, z = oc.drift(x, y, z, d0["vel"], d0["rung"], r, 1.0, t0 * 2.0 ** -r)
+        out["bit_exact_vs_oracle"] = bool(np.array_equal(x, d1["x"]) and np.array_equal(y, d1["y"]) and np.array_equal(z, d1["z"]))
+    return out
 
 
 def copy_calibration(mib):
@@ -269,7 +347,7 @@ def main():
     ap.add_argument("--n", type=int, default=256 ** 3)
     ap.add_argument("--workload", default="clustered", choices=["clustered", "uniform"])
     ap.add_argument("--theta", type=float, default=0.55)
-    ap.add_argument("--bucket", default="128",
+    ap.add_argument("--bucket", default="auto",
                     help="leaf bucket size, or 'auto': time a kick at each of --bucket-candidates on the resident box "
                          "and keep the fastest")
     ap.add_argument("--bucket-candidates", default="64,96,128,160,192")
@@ -284,6 +362,8 @@ def main():
     ap.add_argument("--precision-sweep", action="store_true",
                     help="instead of the throughput line: FP32 compensated / FP64 / FP32 uncompensated kicks at theta "
                          "0.4, 0.55, 0.7 with their rms and p99 force errors")
+    ap.add_argument("--config1", action="store_true",
+                    help="the plain-method case: 64^3 uniform, theta 0.7, GPU and CPU SIMD kicks with their force errors")
     ap.add_argument("--ensemble", action="store_true",
                     help="instead of the throughput line: the 64-box ensemble kicked in one batched, graph-captured "
                          "launch sequence against one box per launch, with the autotuned bucket")
@@ -307,6 +387,9 @@ def main():
         args.bucket = int(args.bucket)
     elif args.ensemble or args.serve:
         args.bucket = 128  # the ensemble mode tunes its own bucket; serving uses the reference default
+    if args.config1:
+        config1(ct, args)
+        return
     if args.ensemble:
         ensemble(ct, args)
         return
@@ -393,10 +476,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(x, y, z, args, h, gpu_forces)
-    # drift timing (reported beside the kick)
-    td = time.perf_counter()
-    box.drift(0, 1.0, 1.0 / 64)
-    drift_ms = 1e3 * (time.perf_counter() - td)
+    drift = drift_check(ct, box, args)
     out = {
         "metric": "particle-kicks/s at rms force err <=1e-3 vs FP64 Ewald; p50/p99 step latency",
         "value": value,
@@ -428,7 +508,7 @@ def main():
         "phases_ms": {k: stats[k] for k in ("walk_ms", "cells_ms", "l2l_ms", "leaf_ms", "pp_ms", "leafkick_ms", "kick_ms")},
         "events": {k: stats[k] for k in ("pp_pairs", "cc", "ewald_cc", "cp_parts", "pc_pairs", "nodes", "sinks", "checks")},
         "tree_build_s": tree_s,
-        "drift_ms": drift_ms,
+        "drift": drift,
     }
     box.close()
     print(json.dumps(out), flush=True)

@@ -878,14 +878,25 @@ __device__ __forceinline__ void load4<float>(const float* p, float& a, float& b,
 	d = v.w;
 }
 template<>
-__device__ __forceinline__ void load4<double>(const double* p, double& a, double& b, double& c, double& d) {
-	const double2 v = *reinterpret_cast<const double2*>(p);
-	const double2 w = *reinterpret_cast<const double2*>(p + 2);
+__device__ __forceinline__ void load4<int32_t>(const int32_t* p, int32_t& a, int32_t& b, int32_t& c, int32_t& d) {
+	const int4 v = *reinterpret_cast<const int4*>(p);
 	a = v.x;
 	b = v.y;
-	c = w.x;
-	d = w.y;
+	c = v.z;
+	d = v.w;
 }
+
+// The P2P source tile's element: the offset as a float in the FP32 modes,
+// as the exact int32 in the FP64 mode (converted per pair; a double tile
+// would double the LDS traffic).
+template<int PR>
+struct PPTile {
+	using type = float;
+};
+template<>
+struct PPTile<kF64> {
+	using type = int32_t;
+};
 
 // One softened pair in integer units (Delta = sink - source in units of 2^-32),
 // g -= Delta/r^3, phi -= 1/r; Ferrers n=4 inside h (cosmictiger/gravity.hpp:76-96).
@@ -959,9 +970,10 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	using Acc = typename Prec<PR>::A;
 	constexpr int S = 64 / P;
 	constexpr int STR = kPPTile / P + 16;
-	__shared__ __align__(16) T tx[P * STR];
-	__shared__ __align__(16) T ty[P * STR];
-	__shared__ __align__(16) T tz[P * STR];
+	using TS = typename PPTile<PR>::type;
+	__shared__ __align__(16) TS tx[P * STR];
+	__shared__ __align__(16) TS ty[P * STR];
+	__shared__ __align__(16) TS tz[P * STR];
 	const int2 le = A.leaves[blockIdx.x];
 	const BoxDesc Bv = load_uniform(A.boxes + uniform(le.x));
 	const BoxDesc* B = &Bv;
@@ -999,9 +1011,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 	const T ih3inv = ih2inv * ihinv;
 	unsigned long long nsrc = 0;
 	int tn = 0;
-	const T* bx = tx + phase * STR;
-	const T* by = ty + phase * STR;
-	const T* bz = tz + phase * STR;
+	const TS* bx = tx + phase * STR;
+	const TS* by = ty + phase * STR;
+	const TS* bz = tz + phase * STR;
 	// near = true: a tile of sources that may lie within h of a sink (checked
 	// pairs with the softening branch); near = false: the branch-free pair.
 	auto flush = [&](auto near_tag) __attribute__((always_inline)) {
@@ -1016,29 +1028,29 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			const T X0 = sx[g], Y0 = sy[g], Z0 = sz[g];
 			for (int t = 0; t < q; t += 4) {
 				struct {
-					T x, y, z, w;
+					TS x, y, z, w;
 				} X4, Y4, Z4;
 				load4(bx + t, X4.x, X4.y, X4.z, X4.w);
 				load4(by + t, Y4.x, Y4.y, Y4.z, Y4.w);
 				load4(bz + t, Z4.x, Z4.y, Z4.z, Z4.w);
 				if constexpr (near) {
-					pp_pair(X0, Y0, Z0, X4.x, Y4.x, Z4.x, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
-					pp_pair(X0, Y0, Z0, X4.y, Y4.y, Z4.y, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
-					pp_pair(X0, Y0, Z0, X4.z, Y4.z, Z4.z, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
-					pp_pair(X0, Y0, Z0, X4.w, Y4.w, Z4.w, h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, T(X4.x), T(Y4.x), T(Z4.x), h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, T(X4.y), T(Y4.y), T(Z4.y), h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, T(X4.z), T(Y4.z), T(Z4.z), h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+					pp_pair(X0, Y0, Z0, T(X4.w), T(Y4.w), T(Z4.w), h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
 				} else {
-					pp_pair_far(X0, Y0, Z0, X4.x, Y4.x, Z4.x, ax, ay, az, ap);
-					pp_pair_far(X0, Y0, Z0, X4.y, Y4.y, Z4.y, ax, ay, az, ap);
-					pp_pair_far(X0, Y0, Z0, X4.z, Y4.z, Z4.z, ax, ay, az, ap);
-					pp_pair_far(X0, Y0, Z0, X4.w, Y4.w, Z4.w, ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, T(X4.x), T(Y4.x), T(Z4.x), ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, T(X4.y), T(Y4.y), T(Z4.y), ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, T(X4.z), T(Y4.z), T(Z4.z), ax, ay, az, ap);
+					pp_pair_far(X0, Y0, Z0, T(X4.w), T(Y4.w), T(Z4.w), ax, ay, az, ap);
 				}
 			}
 			for (int t = q; t < q + 4; t++) {
 				if (t * P + phase < tn) {
 					if constexpr (near) {
-						pp_pair(X0, Y0, Z0, bx[t], by[t], bz[t], h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
+						pp_pair(X0, Y0, Z0, T(bx[t]), T(by[t]), T(bz[t]), h2, ihinv, ih2inv, ih3inv, ax, ay, az, ap);
 					} else {
-						pp_pair_far(X0, Y0, Z0, bx[t], by[t], bz[t], ax, ay, az, ap);
+						pp_pair_far(X0, Y0, Z0, T(bx[t]), T(by[t]), T(bz[t]), ax, ay, az, ap);
 					}
 				}
 			}
@@ -1071,9 +1083,9 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			for (int t = lane; t < m; t += 64) {
 				const int i = tn + t;
 				const int at = (i % P) * STR + i / P;
-				tx[at] = T(int32_t(px[pb + t] - cs0));
-				ty[at] = T(int32_t(py[pb + t] - cs1));
-				tz[at] = T(int32_t(pz[pb + t] - cs2));
+				tx[at] = TS(int32_t(px[pb + t] - cs0));
+				ty[at] = TS(int32_t(py[pb + t] - cs1));
+				tz[at] = TS(int32_t(pz[pb + t] - cs2));
 			}
 			tn += m;
 			pb += m;

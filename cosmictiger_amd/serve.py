@@ -79,9 +79,17 @@ class GpuExecutor:
         self.dt = dt
         self.graph = graph
         self.batch = None  # one batch context for every step: its arenas persist
+        # seconds spent per operation (summed over the host threads that ran them)
+        self.timing = {"create_s": 0.0, "rebuild_s": 0.0, "step_s": 0.0, "steps": 0}
+        self._tlock = threading.Lock()
+
+    def _add(self, key, dt):
+        with self._tlock:
+            self.timing[key] += dt
 
     def create(self, req):
         from . import workloads
+        t0 = time.perf_counter()
         ct = self.ct
         n = req.n
         x, y, z = workloads.nfw_box(n, req.seed, clustered_fraction=req.clustered, nhalos=max(8, n // 4096))
@@ -89,10 +97,13 @@ class GpuExecutor:
         box = ct.Box(x, y, z, vel, np.zeros(n, np.int8))
         h = ct.softening(n)
         box.tree_build(self.theta, self.bucket, h)
+        self._add("create_s", time.perf_counter() - t0)
         return {"box": box, "h": h, "n": n, "step": 0}
 
     def rebuild(self, st):
+        t0 = time.perf_counter()
         st["box"].tree_build(self.theta, self.bucket, st["h"])
+        self._add("rebuild_s", time.perf_counter() - t0)
 
     def params(self, st):
         p = self.ct.default_kick_params(self.theta, st["h"], 1.0 / st["n"])
@@ -103,6 +114,7 @@ class GpuExecutor:
 
     def step(self, states):
         """One kick + drift of every box in states, together."""
+        t0 = time.perf_counter()
         ct = self.ct
         boxes = [s["box"] for s in states]
         if self.batch is None:
@@ -112,12 +124,17 @@ class GpuExecutor:
         self.batch.step([self.params(s) for s in states], 0, [1.0] * len(states), [self.dt] * len(states), graph=self.graph)
         for s in states:
             s["step"] += 1
+        self._add("step_s", time.perf_counter() - t0)
+        self._add("steps", 1)
 
     def step_alone(self, st):
         """The same step with the box's own launches (one box per launch)."""
+        t0 = time.perf_counter()
         st["box"].kick(self.params(st))
         st["box"].drift(0, 1.0, self.dt)
         st["step"] += 1
+        self._add("step_s", time.perf_counter() - t0)
+        self._add("steps", 1)
 
     def read(self, st):
         return st["box"].read()
