@@ -174,8 +174,15 @@ def serving(ct, args):
     budget = 64 * 2 ** 30
     nreq = args.serve_requests
 
+    # the requests' particles are the clients' data: made once, before the runs
+    base = serve.poisson_requests(nreq, 1.0, seed=77)
+    for r in base:
+        serve.make_particles(r)
+
     def run(rate, batched):
         reqs = serve.poisson_requests(nreq, rate, seed=77)
+        for r, b in zip(reqs, base):
+            r.particles = b.particles
         srv = serve.Server(ex, hbm_budget=budget, host_threads=min(16, os.cpu_count() or 1), batched=batched)
         for k in ex.timing:
             ex.timing[k] = 0

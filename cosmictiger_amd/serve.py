@@ -55,6 +55,21 @@ class Request:
     cancelled: bool = False
     rejected: bool = False
     result: Optional[dict] = None
+    # the request's particles (x, y, z, vel), as a client sends them; made on
+    # first use when absent (make_particles)
+    particles: Optional[tuple] = None
+
+
+def make_particles(req):
+    """The synthetic box of a request: NFW halos plus a uniform background
+    (clustered fraction req.clustered), Gaussian velocities, seeded."""
+    from . import workloads
+    if req.particles is None:
+        n = req.n
+        x, y, z = workloads.nfw_box(n, req.seed, clustered_fraction=req.clustered, nhalos=max(8, n // 4096))
+        vel = workloads.gaussian_velocities(n, req.seed, sigma=1e-3)
+        req.particles = (x, y, z, vel)
+    return req.particles
 
 
 def poisson_requests(count, rate, seed=0, nmin=16 ** 3, nmax=192 ** 3, max_steps=10):
@@ -88,12 +103,10 @@ class GpuExecutor:
             self.timing[key] += dt
 
     def create(self, req):
-        from . import workloads
+        x, y, z, vel = make_particles(req)
         t0 = time.perf_counter()
         ct = self.ct
         n = req.n
-        x, y, z = workloads.nfw_box(n, req.seed, clustered_fraction=req.clustered, nhalos=max(8, n // 4096))
-        vel = workloads.gaussian_velocities(n, req.seed, sigma=1e-3)
         box = ct.Box(x, y, z, vel, np.zeros(n, np.int8))
         h = ct.softening(n)
         box.tree_build(self.theta, self.bucket, h)
