@@ -223,7 +223,10 @@ __device__ __forceinline__ WalkCounts walk_count(const DNode* __restrict__ nodes
 
 // reservation slots of one entry
 enum { R_E = 0, R_CC, R_CP, R_PC, R_PP, R_ACT, R_DP, R_EP, R_SEG, R_NUM };
-constexpr int kWalkBatch = 16;
+constexpr int kWalkBatch = 64;
+// waves per SIMD the walk is compiled for (<= 96 VGPRs); the walk is bound by
+// the latency of its dependent loads, so resident waves are what hides it
+constexpr int kWalkWaves = 5;
 
 // One level of the dual-list walk (src/kick.cu:239-484).  A wave takes
 // kWalkBatch (sink node, check lists) entries at a time: pass 1 classifies
@@ -233,7 +236,7 @@ constexpr int kWalkBatch = 16;
 // check lists of the next level and the entries of the next level.  Writes
 // keep the order of the check list, so lists match a serial walk entry for
 // entry.
-__global__ __launch_bounds__(256) void k_walk(WalkArgs A, int k) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves))) void k_walk(WalkArgs A, int k) {
 	Counters* C = A.cnt;
 	if (__builtin_amdgcn_readfirstlane(C->overflow | C->walk_error)) return;
 	const int nact = uniform(min(C->act[k], int(A.cap_act)));
@@ -1835,7 +1838,25 @@ WalkArgs walk_args(KickCtx* c) {
 	return A;
 }
 
-constexpr int kWalkGrid = 1280;  // 5120 waves: 5 per SIMD, all resident at the walk's register use (<= 102 VGPRs)
+// Workgroups of k_walk resident at once on the library's device (ct_init):
+// every CU at the occupancy that the kernel's resources allow.  A larger grid
+// would leave a second round of waves behind the first in the walk's
+// grid-stride loop.
+int walk_grid() {
+	static std::atomic<int> grid{0};
+	int g = grid.load(std::memory_order_relaxed);
+	if (g == 0) {
+		int dev = 0, ncu = 0, per_cu = 0;
+		if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+				hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk, 256, 0) == hipSuccess && ncu > 0 && per_cu > 0) {
+			g = ncu * per_cu;
+		} else {
+			g = 256 * kWalkWaves;
+		}
+		grid.store(g, std::memory_order_relaxed);
+	}
+	return g;
+}
 
 int walk_iters(KickCtx* c) {
 	return c->max_depth + 3;
@@ -1855,8 +1876,9 @@ void launch_walk(KickCtx* c, hipStream_t st) {
 	hipLaunchKernelGGL(k_walk_seed_counts, dim3(1), dim3(64), 0, st, c->cnt.p, c->nbox());
 	const WalkArgs A = walk_args(c);
 	const int iters = walk_iters(c);
+	const int grid = walk_grid();
 	for (int k = 0; k < iters; k++) {
-		hipLaunchKernelGGL(k_walk, dim3(kWalkGrid), dim3(256), 0, st, A, k);
+		hipLaunchKernelGGL(k_walk, dim3(grid), dim3(256), 0, st, A, k);
 	}
 }
 
