@@ -80,13 +80,20 @@ enum ListKind { kEwald = 0, kCC = 1, kCP = 2, kPC = 3, kPP = 4, kNumLists = 5 };
 
 constexpr int kMaxIter = 300;
 
+// A counter alone in its 256-byte block: reservation counters that share a
+// cache line serialize each other's atomics.
+struct __align__(256) HotCounter {
+	int32_t v;
+	int32_t pad[63];
+};
+
 // Counter block in device memory (int32 unless noted).
 struct Counters {
 	int32_t act[kMaxIter + 2];
 	int32_t dp[kMaxIter + 2];
 	int32_t ep[kMaxIter + 2];
-	int32_t out[kNumLists];
-	int32_t seg;
+	HotCounter out[kNumLists];
+	HotCounter seg;
 	int32_t overflow;   // bit k set: buffer k overflowed
 	int32_t walk_error;
 	unsigned long long stat[8];  // event counters of the kick

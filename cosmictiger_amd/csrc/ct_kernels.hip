@@ -230,12 +230,13 @@ constexpr int kWalkWaves = 5;
 
 // One level of the dual-list walk (src/kick.cu:239-484).  A wave takes
 // kWalkBatch (sink node, check lists) entries at a time: pass 1 classifies
-// every check of every entry and counts per list; a wave prefix scan turns the
-// counts into offsets and lane 0 reserves the batch's space with one atomic
-// per list; pass 2 classifies again and writes the interaction lists, the
-// check lists of the next level and the entries of the next level.  Writes
-// keep the order of the check list, so lists match a serial walk entry for
-// entry.
+// every check of every entry, keeps each class and counts per list; a wave
+// prefix scan turns the counts into offsets and lanes 0..8 reserve the batch's
+// space, one atomic per list, in one instruction (each counter in its own
+// 256-byte block: counters sharing a line serialize the atomics of every
+// wave); pass 2 writes the interaction lists, the check lists of the next
+// level and the entries of the next level from the kept classes.  Writes keep
+// the order of the check list, so lists match a serial walk entry for entry.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves))) void k_walk(WalkArgs A, int k) {
 	Counters* C = A.cnt;
 	if (__builtin_amdgcn_readfirstlane(C->overflow | C->walk_error)) return;
@@ -296,10 +297,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 				my_skip = 0;
 			}
 		}
-		// ---- wave scan, one reservation per list for the batch
+		// ---- wave scans, then one reservation per list for the batch: lane q
+		// reserves list q, so the R_NUM atomics are one instruction and one
+		// round trip
 		int excl[R_NUM];
 		int base[R_NUM];
 		bool ok[R_NUM];
+		int want = 0;
 #pragma unroll
 		for (int q = 0; q < R_NUM; q++) {
 			int v = my[q];
@@ -309,28 +313,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 			}
 			excl[q] = v - my[q];
 			const int tot = __builtin_amdgcn_readlane(v, 63);
+			want = lane == q ? tot : want;
+		}
+		{
 			int r = 0;
-			bool okq = true;
-			if (lane == 0 && tot > 0) {
+			int okq = 1;
+			if (lane < R_NUM && want > 0) {
 				int* ctr;
 				int64_t cap;
-				int bit;
-				switch (q) {
-				case R_E: ctr = &C->out[kEwald]; cap = A.cap_out[kEwald]; bit = 0; break;
-				case R_CC: ctr = &C->out[kCC]; cap = A.cap_out[kCC]; bit = 1; break;
-				case R_CP: ctr = &C->out[kCP]; cap = A.cap_out[kCP]; bit = 2; break;
-				case R_PC: ctr = &C->out[kPC]; cap = A.cap_out[kPC]; bit = 3; break;
-				case R_PP: ctr = &C->out[kPP]; cap = A.cap_out[kPP]; bit = 4; break;
-				case R_ACT: ctr = &C->act[k + 1]; cap = A.cap_act; bit = 5; break;
-				case R_DP: ctr = &C->dp[k + 1]; cap = A.cap_dp; bit = 6; break;
-				case R_EP: ctr = &C->ep[k + 1]; cap = A.cap_ep; bit = 7; break;
-				default: ctr = &C->seg; cap = A.cap_seg; bit = 8; break;
+				switch (lane) {
+				case R_E: ctr = &C->out[kEwald].v; cap = A.cap_out[kEwald]; break;
+				case R_CC: ctr = &C->out[kCC].v; cap = A.cap_out[kCC]; break;
+				case R_CP: ctr = &C->out[kCP].v; cap = A.cap_out[kCP]; break;
+				case R_PC: ctr = &C->out[kPC].v; cap = A.cap_out[kPC]; break;
+				case R_PP: ctr = &C->out[kPP].v; cap = A.cap_out[kPP]; break;
+				case R_ACT: ctr = &C->act[k + 1]; cap = A.cap_act; break;
+				case R_DP: ctr = &C->dp[k + 1]; cap = A.cap_dp; break;
+				case R_EP: ctr = &C->ep[k + 1]; cap = A.cap_ep; break;
+				default: ctr = &C->seg.v; cap = A.cap_seg; break;
 				}
-				r = atomicAdd(ctr, tot);
-				okq = reserve_ok(C, bit, r, tot, cap);
+				r = atomicAdd(ctr, want);
+				okq = reserve_ok(C, lane, r, want, cap);
 			}
-			base[q] = uniform(r);
-			ok[q] = uniform(int(okq));
+#pragma unroll
+			for (int q = 0; q < R_NUM; q++) {
+				base[q] = __builtin_amdgcn_readlane(r, q);
+				ok[q] = __builtin_amdgcn_readlane(okq, q);
+			}
 		}
 		// ---- pass 2: writes
 		for (int e = 0; e < nb; e++) {
@@ -1909,8 +1918,8 @@ int run_walk(KickCtx* c, hipStream_t st) {
 		if (C.overflow & (1 << 6)) bump(c->cap_dp, maxd);
 		if (C.overflow & (1 << 7)) bump(c->cap_ep, maxe);
 		for (int q = 0; q < kNumLists; q++)
-			if (C.overflow & (1 << q)) bump(c->cap_out[q], C.out[q]);
-		if (C.overflow & (1 << 8)) bump(c->cap_seg, C.seg);
+			if (C.overflow & (1 << q)) bump(c->cap_out[q], C.out[q].v);
+		if (C.overflow & (1 << 8)) bump(c->cap_seg, C.seg.v);
 	}
 	return fail(CT_E_ARENA, "walk: list buffers kept overflowing");
 }
@@ -2567,12 +2576,12 @@ int ct_lists_read(ct_box_t b, int kind, int64_t* off, int32_t* idx, int64_t idx_
 	const int64_t nn = b->nnodes;
 	std::vector<int32_t> head(std::max<int64_t>(nn, 1));
 	const Counters& C = c->last_counters;
-	std::vector<Seg> seg(std::max(1, C.seg));
-	std::vector<int32_t> out(std::max(1, C.out[kind]));
+	std::vector<Seg> seg(std::max(1, C.seg.v));
+	std::vector<int32_t> out(std::max(1, C.out[kind].v));
 	CT_HIP(hipDeviceSynchronize());
 	if (nn) CT_HIP(hipMemcpy(head.data(), c->head.p + int64_t(kind) * nn, nn * 4, hipMemcpyDeviceToHost));
-	if (C.seg) CT_HIP(hipMemcpy(seg.data(), c->seg.p, C.seg * sizeof(Seg), hipMemcpyDeviceToHost));
-	if (C.out[kind]) CT_HIP(hipMemcpy(out.data(), c->out[kind].p, int64_t(C.out[kind]) * 4, hipMemcpyDeviceToHost));
+	if (C.seg.v) CT_HIP(hipMemcpy(seg.data(), c->seg.p, C.seg.v * sizeof(Seg), hipMemcpyDeviceToHost));
+	if (C.out[kind].v) CT_HIP(hipMemcpy(out.data(), c->out[kind].p, int64_t(C.out[kind].v) * 4, hipMemcpyDeviceToHost));
 	int64_t tot = 0;
 	for (int64_t i = 0; i < nn; i++) {
 		off[i] = tot;

@@ -120,6 +120,15 @@ __global__ void k_split_flags(const TNode* __restrict__ nodes, int lb, int le, c
 
 // Children of the splitting nodes of one level, written at the next level in
 // breadth-first order (the scan of the split flags).
+// The number of nodes of a level that split (the scan's last entry plus the
+// last flag) and the depth error flag, side by side for one host read.
+__global__ void k_level_count(const int32_t* __restrict__ scan, const int32_t* __restrict__ flag, int cnt, int32_t* __restrict__ err) {
+	if (threadIdx.x == 0) {
+		err[1] = scan[cnt - 1] + flag[cnt - 1];
+		err[2] = err[0];
+	}
+}
+
 __global__ void k_children(TNode* __restrict__ nodes, int lb, int le, const int32_t* __restrict__ flag, const int32_t* __restrict__ scan,
 		const uint64_t* __restrict__ key, int koff, int next) {
 	const int q = lb + blockIdx.x * blockDim.x + threadIdx.x;
@@ -590,7 +599,7 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 	int64_t cap = std::max<int64_t>(4 * int64_t(n) / std::max(1, bucket / 2) + 1024, 1024);
 	auto& tn = S.tn;
 	cap = std::max<int64_t>(cap, tn.n);
-	if ((rc = ewd.grow(int64_t(ew.size()))) || (rc = err.grow(1)) || (rc = tn.grow(cap))) {
+	if ((rc = ewd.grow(int64_t(ew.size()))) || (rc = err.grow(3)) || (rc = tn.grow(cap))) {
 		cleanup();
 		return rc;
 	}
@@ -625,15 +634,14 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 			cleanup();
 			return fail(CT_E_HIP, "gpu tree: scan failed");
 		}
-		int32_t last_scan = 0, last_flag = 0, errv = 0;
-		CT_HIP(hipMemcpy(&last_scan, scan.p + cnt - 1, 4, hipMemcpyDeviceToHost));
-		CT_HIP(hipMemcpy(&last_flag, flag.p + cnt - 1, 4, hipMemcpyDeviceToHost));
-		CT_HIP(hipMemcpy(&errv, err.p, 4, hipMemcpyDeviceToHost));
-		if (errv) {
+		hipLaunchKernelGGL(k_level_count, dim3(1), dim3(64), 0, st, scan.p, flag.p, cnt, err.p);
+		int32_t lc[2] = {0, 0};
+		CT_HIP(hipMemcpy(lc, err.p + 1, sizeof(lc), hipMemcpyDeviceToHost));
+		if (lc[1]) {
 			cleanup();
 			return fail(CT_E_TREE, "gpu tree: a node deeper than 63 levels must split");
 		}
-		const int nsplit = last_scan + last_flag;
+		const int nsplit = lc[0];
 		const int next = le + 2 * nsplit;
 		if (next > cap) {
 			// grow the node arena, keeping what is built
