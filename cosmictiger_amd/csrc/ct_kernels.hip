@@ -684,8 +684,8 @@ template<class T>
 __global__ __launch_bounds__(64) void k_cells_ewald(CellArgs A) {
 	const int lane = threadIdx.x;
 	unsigned long long nev = 0;
-	// one node per workgroup (the dispatcher balances the very uneven list
-	// lengths); the box descriptor is kept in scalar registers
+	// a node at a time per workgroup; the box descriptor is kept in scalar
+	// registers
 	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
 		const int bx = uniform(find_box(A.node_off, A.nbox, g));
 		const BoxDesc Bv = load_uniform(A.boxes + bx);
@@ -733,8 +733,8 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 	__shared__ uint32_t tx[kCellTile], ty[kCellTile], tz[kCellTile];
 	const int lane = threadIdx.x;
 	unsigned long long ncc = 0, ncp = 0;
-	// one node per workgroup (the dispatcher balances the very uneven list
-	// lengths); the box descriptor is kept in scalar registers
+	// a node at a time per workgroup; the box descriptor is kept in scalar
+	// registers
 	for (int g = blockIdx.x; g < A.total; g += gridDim.x) {
 		const int bx = uniform(find_box(A.node_off, A.nbox, g));
 		const BoxDesc Bv = load_uniform(A.boxes + bx);
@@ -1847,23 +1847,38 @@ WalkArgs walk_args(KickCtx* c) {
 	return A;
 }
 
-// Workgroups of k_walk resident at once on the library's device (ct_init):
-// every CU at the occupancy that the kernel's resources allow.  A larger grid
-// would leave a second round of waves behind the first in the walk's
-// grid-stride loop.
+// Workgroups of `kernel` (`block` threads, static LDS only) resident at once
+// on the library's device (ct_init): every CU at the occupancy that the
+// kernel's resources allow, or `fallback` if the runtime cannot say.
+template<class K>
+int resident_blocks(K kernel, int block, int fallback) {
+	int dev = 0, ncu = 0, per_cu = 0;
+	if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+			hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) == hipSuccess && ncu > 0 && per_cu > 0)
+		return ncu * per_cu;
+	return fallback;
+}
+
+// The walk's grid: a larger one would leave a second round of waves behind
+// the first in its grid-stride loop.
 int walk_grid() {
-	static std::atomic<int> grid{0};
-	int g = grid.load(std::memory_order_relaxed);
-	if (g == 0) {
-		int dev = 0, ncu = 0, per_cu = 0;
-		if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-				hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk, 256, 0) == hipSuccess && ncu > 0 && per_cu > 0) {
-			g = ncu * per_cu;
-		} else {
-			g = 256 * kWalkWaves;
-		}
-		grid.store(g, std::memory_order_relaxed);
-	}
+	static const int g = resident_blocks(k_walk, 256, 256 * kWalkWaves);
+	return g;
+}
+
+// Grids of the cell kernels: 8 workgroups per resident slot, each looping
+// over nodes.  A workgroup per node spends time being dispatched on the many
+// nodes with short lists; one per slot leaves the slowest slot's nodes as a
+// tail.
+constexpr int kCellGridPerSlot = 8;
+template<class T>
+int cell_grid_ewald() {
+	static const int g = kCellGridPerSlot * resident_blocks(k_cells_ewald<T>, 64, 4096);
+	return g;
+}
+template<class T, bool CP>
+int cell_grid_direct() {
+	static const int g = kCellGridPerSlot * resident_blocks(k_cells_direct<T, CP>, 64, 4096);
 	return g;
 }
 
@@ -1941,10 +1956,10 @@ void launch_interactions(KickCtx* c, hipStream_t st, bool events) {
 	ca.out_cc = c->out[kCC].p;
 	ca.out_cp = c->out[kCP].p;
 	ca.cnt = c->cnt.p;
-	const int ng = std::max(1, std::min(c->total_nodes, 1 << 24));
-	hipLaunchKernelGGL((k_cells_ewald<T>), dim3(ng), dim3(64), 0, st, ca);
-	hipLaunchKernelGGL((k_cells_direct<T, false>), dim3(ng), dim3(64), 0, st, ca);
-	hipLaunchKernelGGL((k_cells_direct<T, true>), dim3(ng), dim3(64), 0, st, ca);
+	const int nn = std::max(1, c->total_nodes);
+	hipLaunchKernelGGL((k_cells_ewald<T>), dim3(std::min(nn, cell_grid_ewald<T>())), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL((k_cells_direct<T, false>), dim3(std::min(nn, cell_grid_direct<T, false>())), dim3(64), 0, st, ca);
+	hipLaunchKernelGGL((k_cells_direct<T, true>), dim3(std::min(nn, cell_grid_direct<T, true>())), dim3(64), 0, st, ca);
 	if (events) (void) hipEventRecord(c->ev[2], st);
 	for (int d = 1; d <= c->max_depth; d++) {
 		const int tot = c->lvl_total[d];
