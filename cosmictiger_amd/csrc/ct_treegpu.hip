@@ -474,7 +474,14 @@ struct TreeScratch {
 	DBuf<int32_t> err, flag, scan;
 	DBuf<TNode> tn;
 	DBuf<GNode> geo;
+	// The builds' own stream, non-blocking: a box's rebuild does not wait for
+	// (or hold up) work that other host threads put on the null stream.
+	hipStream_t st = nullptr;
 	void release() {
+		if (st) {
+			(void) hipStreamDestroy(st);
+			st = nullptr;
+		}
 		kin.release();
 		kout.release();
 		iin.release();
@@ -516,7 +523,8 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 	// the tree covers the active range [rb, n) (particles_current_range)
 	const int rb = int(b->rung_begin);
 	const int m = n - rb;
-	hipStream_t st = nullptr;
+	if (!S.st) CT_HIP(hipStreamCreateWithFlags(&S.st, hipStreamNonBlocking));
+	hipStream_t st = S.st;
 	int rc;
 	auto& kin = S.kin;
 	auto& kout = S.kout;
@@ -603,19 +611,25 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		cleanup();
 		return rc;
 	}
-	CT_HIP(hipMemcpy(ewd.p, ew.data(), ew.size(), hipMemcpyHostToDevice));
-	CT_HIP(hipMemset(err.p, 0, 4));
-	{
-		TNode root;
-		root.prefix = 0;
-		root.pbegin = rb;
-		root.pend = n;
-		root.depth = 0;
-		root.parent = -1;
-		root.child0 = -1;
-		root.pad = 0;
-		CT_HIP(hipMemcpy(tn.p, &root, sizeof(TNode), hipMemcpyHostToDevice));
-	}
+	// (host sources of the asynchronous copies live until the build's final
+	// synchronisation)
+	TNode root;
+	root.prefix = 0;
+	root.pbegin = rb;
+	root.pend = n;
+	root.depth = 0;
+	root.parent = -1;
+	root.child0 = -1;
+	root.pad = 0;
+	// declared after ew and root, so that every return waits for the stream
+	// before they go
+	struct StreamWait {
+		hipStream_t s;
+		~StreamWait() { (void) hipStreamSynchronize(s); }
+	} wait_on_return{st};
+	CT_HIP(hipMemcpyAsync(ewd.p, ew.data(), ew.size(), hipMemcpyHostToDevice, st));
+	CT_HIP(hipMemsetAsync(err.p, 0, 4, st));
+	CT_HIP(hipMemcpyAsync(tn.p, &root, sizeof(TNode), hipMemcpyHostToDevice, st));
 	for (int d = 0;; d++) {
 		const int lb = level_off[d], le = level_off[d + 1];
 		const int cnt = le - lb;
@@ -636,7 +650,8 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 		}
 		hipLaunchKernelGGL(k_level_count, dim3(1), dim3(64), 0, st, scan.p, flag.p, cnt, err.p);
 		int32_t lc[2] = {0, 0};
-		CT_HIP(hipMemcpy(lc, err.p + 1, sizeof(lc), hipMemcpyDeviceToHost));
+		CT_HIP(hipMemcpyAsync(lc, err.p + 1, sizeof(lc), hipMemcpyDeviceToHost, st));
+		CT_HIP(hipStreamSynchronize(st));
 		if (lc[1]) {
 			cleanup();
 			return fail(CT_E_TREE, "gpu tree: a node deeper than 63 levels must split");
@@ -651,7 +666,8 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 				cleanup();
 				return rc;
 			}
-			CT_HIP(hipMemcpy(bigger.p, tn.p, int64_t(le) * sizeof(TNode), hipMemcpyDeviceToDevice));
+			CT_HIP(hipMemcpyAsync(bigger.p, tn.p, int64_t(le) * sizeof(TNode), hipMemcpyDeviceToDevice, st));
+			CT_HIP(hipStreamSynchronize(st));
 			tn.release();
 			tn.p = bigger.p;
 			tn.n = bigger.n;
@@ -682,7 +698,8 @@ int gpu_tree_build(ct_box* b, double theta, int bucket, double hsoft, GpuTreeRes
 	CT_HIP(hipGetLastError());
 	// host bookkeeping: per-node (pbegin, pend, leaf) for the leaf lists
 	std::vector<TNode> ht(nn);
-	CT_HIP(hipMemcpy(ht.data(), tn.p, int64_t(nn) * sizeof(TNode), hipMemcpyDeviceToHost));
+	CT_HIP(hipMemcpyAsync(ht.data(), tn.p, int64_t(nn) * sizeof(TNode), hipMemcpyDeviceToHost, st));
+	CT_HIP(hipStreamSynchronize(st));
 	res->nnodes = nn;
 	res->max_depth = max_depth;
 	res->level_off = level_off;
