@@ -455,7 +455,7 @@ def main():
         try:
             with open(args.traffic_file) as f:
                 tr = json.load(f)
-            if tr.get("n") == n and tr.get("workload") == args.workload:
+            if tr.get("n") == n and tr.get("workload") == args.workload and tr.get("bucket") == args.bucket:
                 traffic = tr.get("bytes_per_launch")
         except Exception:
             traffic = None

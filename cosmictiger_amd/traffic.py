@@ -2,7 +2,7 @@
 
     rocprofv3 --pmc FETCH_SIZE -f csv -d DIR_F -o run -- python3 bench.py ... --copy-calibration-mib M
     rocprofv3 --pmc WRITE_SIZE -f csv -d DIR_W -o run -- python3 bench.py ... --copy-calibration-mib M
-    python3 -m cosmictiger_amd.traffic DIR_F DIR_W --copy-mib M --n N --workload clustered --out FILE.json
+    python3 -m cosmictiger_amd.traffic DIR_F DIR_W --copy-mib M --n N --workload clustered --bucket B --out FILE.json
 
 FETCH_SIZE and WRITE_SIZE are the derived TCC counters (KiB per dispatch).
 The calibration dispatch is the HIP runtime's copy kernel of a device-to-device
@@ -70,12 +70,13 @@ def main(argv=None):
     ap.add_argument("--copy-mib", type=float, required=True)
     ap.add_argument("--n", type=int, required=True)
     ap.add_argument("--workload", default="clustered")
+    ap.add_argument("--bucket", type=int, required=True)
     ap.add_argument("--command", default="")
     ap.add_argument("--out", default="")
     a = ap.parse_args(argv)
     res = summarize(a.fetch_dir, a.write_dir, a.copy_mib)
     total = res["fetch"]["pp_bytes_per_kick"] + res["write"]["pp_bytes_per_kick"]
-    out = {"n": a.n, "workload": a.workload, "kernel": "k_pp (all leaf-size classes of one kick)",
+    out = {"n": a.n, "workload": a.workload, "bucket": a.bucket, "kernel": "k_pp (all leaf-size classes of one kick)",
            "bytes_per_launch": total, "fetch": res["fetch"], "write": res["write"], "command": a.command}
     s = json.dumps(out, indent=1)
     if a.out:
