@@ -281,6 +281,41 @@ def drift_check(ct, box, args):
         for r in rungs:
             x, y, z = oc.drift(x, y, z, d0["vel"], d0["rung"], r, 1.0, t0 * 2.0 ** -r)
         out["bit_exact_vs_oracle"] = bool(np.array_equal(x, d1["x"]) and np.array_equal(y, d1["y"]) and np.array_equal(z, d1["z"]))
+    out["lightcone"] = lightcone_check(ct, box, args, rungs[0], t0 * 2.0 ** -rungs[0])
+    return out
+
+
+def lightcone_check(ct, box, args, rung, dt):
+    """The drift with the lightcone (src/drift.cu:36-154, do_lc) of the
+    particles of one rung, with the light shell at radius ~0.8 from the
+    observer: the first call finds its entries (and sizes the entry buffer),
+    the second, with the next conformal-time window, is timed.  Algorithmic
+    bytes: 25 B per particle read by the crossing pass, 37 B per particle by
+    the drift, 28 B per entry written (the entries stay on the device; the
+    timed call returns their number only).  With the CPU baseline on, the first
+    call's entries are compared with the oracle's (positions and velocities
+    bit for bit; the fraction of equal HEALPix pixels)."""
+    nside = 1024
+    tau_max = 1.5
+    tau1 = tau_max - 0.8
+    d0 = box.read() if not args.no_cpu_baseline else None
+    e1 = box.drift_lightcone(rung, 1.0, dt, tau1 - dt, tau1, tau_max, nside)
+    td = time.perf_counter()
+    n2 = box.drift_lightcone(rung, 1.0, dt, tau1, tau1 + dt, tau_max, nside, read=False)
+    ms = 1e3 * (time.perf_counter() - td)
+    n = box.n
+    nbytes = 62.0 * n + 28.0 * n2
+    out = {"rung": rung, "nside": nside, "entries": [int(len(e1)), int(n2)], "ms": ms, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+           "hbm_frac": nbytes / (ms * 1e-3) / 8e12}
+    if d0 is not None:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ct_oracle as oc
+        w = oc.lightcone(d0["x"], d0["y"], d0["z"], d0["vel"], d0["rung"], rung, 1.0, dt, tau1 - dt, tau1, tau_max, nside)
+        key = lambda e: np.lexsort((e["z"], e["y"], e["x"]))
+        g, w = e1[key(e1)], w[key(w)]
+        same = len(g) == len(w) and all(np.array_equal(g[f], w[f]) for f in ("x", "y", "z", "vx", "vy", "vz"))
+        out["entries_bit_exact_vs_oracle"] = bool(same)
+        out["pixel_agreement"] = float(np.mean(g["pix"] == w["pix"])) if same and len(g) else None
     return out
 
 

@@ -98,6 +98,10 @@ assert NODE_DTYPE.itemsize == ctypes.sizeof(Node)
 _P = ctypes.c_void_p
 _I64P = ctypes.POINTER(ctypes.c_int64)
 
+# ct_lc_entry (lc_entry of cosmictiger/lightcone.hpp:54-58)
+LC_ENTRY = np.dtype([("x", np.int32), ("y", np.int32), ("z", np.int32), ("vx", np.float32), ("vy", np.float32), ("vz", np.float32),
+                     ("pix", np.int32)])
+
 
 class StepParams(ctypes.Structure):
     """Arguments of one call of the hierarchical step (kick_step_hierarchical,
@@ -140,6 +144,9 @@ def _declare(L):
     L.ct_kick.argtypes = [_P, ctypes.POINTER(KickParams), ctypes.POINTER(KickReturn), _P]
     L.ct_kick_prec.argtypes = [_P, ctypes.POINTER(KickParams), ctypes.c_int, ctypes.POINTER(KickReturn), _P]
     L.ct_drift.argtypes = [_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_double, _P]
+    L.ct_drift_lightcone.argtypes = [_P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_int, ctypes.c_double, _I64P, _P]
+    L.ct_lc_read.argtypes = [_P, _P, ctypes.c_int64, _I64P]
     L.ct_box_size.argtypes = [_P, _I64P, _I64P]
     L.ct_box_read.argtypes = [_P] + [_P] * 10
     L.ct_tree_read.argtypes = [_P, _P, ctypes.c_int64]
@@ -173,7 +180,7 @@ def _declare(L):
     for name in ("ct_init", "ct_ewald_init", "ct_box_create", "ct_box_destroy", "ct_tree_build", "ct_tree_build_ex",
                  "ct_tree_build_host", "ct_kick",
                  "ct_kick_prec",
-                 "ct_drift", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
+                 "ct_drift", "ct_drift_lightcone", "ct_lc_read", "ct_box_size", "ct_box_read", "ct_tree_read", "ct_lists_read", "ct_kick_stats", "ct_fmm_host",
                  "ct_fmm_device", "ct_direct_ewald", "ct_batch_create", "ct_batch_destroy", "ct_batch_reset", "ct_batch_step", "ct_batch_kick", "ct_batch_drift",
                  "ct_batch_stats", "ct_box_sort_by_rung", "ct_box_current_range", "ct_box_push_rungs", "ct_box_pop_rungs",
                  "ct_box_set_minrung", "ct_box_rung_counts", "ct_step_hierarchical", "ct_box_save", "ct_box_load"):
@@ -318,6 +325,19 @@ class Box:
     def drift(self, rung, a, dt, drag=0.0, stream=None):
         """cuda_drift over the active range (src/drift.cu:156-195)."""
         _check(lib().ct_drift(self._h, int(rung), float(a), float(dt), float(drag), stream))
+
+    def drift_lightcone(self, rung, a, dt, tau0, tau1, tau_max, nside, drag=0.0, stream=None, read=True):
+        """The drift with the lightcone of cuda_drift_kernel (src/drift.cu:36-154):
+        returns the crossings as an array of LC_ENTRY (in no particular order),
+        or only their number with read=False (they stay with the box)."""
+        n = ctypes.c_int64(0)
+        _check(lib().ct_drift_lightcone(self._h, int(rung), float(a), float(dt), float(tau0), float(tau1), float(tau_max), int(nside),
+                                        float(drag), ctypes.byref(n), stream))
+        if not read:
+            return n.value
+        out = np.zeros(max(n.value, 1), dtype=LC_ENTRY)
+        _check(lib().ct_lc_read(self._h, out.ctypes.data_as(ctypes.c_void_p), n.value, None))
+        return out[:n.value]
 
     # ---- active range and rungs (src/particles.cpp:218-235,583-649,1020-1068)
 

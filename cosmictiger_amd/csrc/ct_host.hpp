@@ -94,4 +94,9 @@ struct ct_box {
 	std::vector<int64_t> rung_stack;
 	// device tree-build scratch kept between builds (ct_treegpu.hip)
 	void* tree_scratch = nullptr;
+	// lightcone entries of the last ct_drift_lightcone (grow-only) and their
+	// device counter
+	ct::DBuf<ct_lc_entry> lc;
+	ct::DBuf<unsigned long long> lc_cnt;
+	int64_t lc_count = 0;
 };

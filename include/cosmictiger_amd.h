@@ -190,6 +190,36 @@ int ct_batch_stats(ct_batch_t batch, double* out, int n);
  * src/drift.cu:156) as called by drift() (src/drift.cpp:33). */
 int ct_drift(ct_box_t box, int rung, float a, float dt, double drag, void* stream);
 
+/* ---- lightcone ---- */
+
+/* One particle crossing the lightcone: lc_entry (cosmictiger/lightcone.hpp:
+ * 54-58).  x, y, z are the crossing position relative to the observer at the
+ * origin as fixed<int,31> raw words (position * 2^31, truncated); vx, vy, vz
+ * the particle's velocity; pix its HEALPix nested pixel. */
+typedef struct {
+	int32_t x, y, z;
+	float vx, vy, vz;
+	int32_t pix;
+} ct_lc_entry;
+
+/* The drift of ct_drift with the lightcone of cuda_drift_kernel
+ * (src/drift.cu:36-154, do_lc = true): for the particles of rung `rung` in the
+ * active range, every periodic image (offsets -1 and 0 per axis) whose
+ * conformal-time shell index floor((tau + |X|) / tau_max) changes over the step
+ * and that lies within the unit sphere yields an entry at its crossing of the
+ * light shell.  Only in the last unit of conformal time (tau_max - tau1 < 1),
+ * as in the reference.  nside: HEALPix resolution, a power of 2 in
+ * [1, 8192].  The entries stay with the box (ct_lc_read); *nentries (may be
+ * NULL) receives their number.  a, dt, tau0, tau1, tau_max are float as
+ * cuda_drift takes them (cosmictiger/drift.hpp:54). */
+int ct_drift_lightcone(ct_box_t box, int rung, float a, float dt, float tau0, float tau1, float tau_max, int nside, double drag,
+		int64_t* nentries, void* stream);
+
+/* Copies min(cap, count) entries of the box's last ct_drift_lightcone to out
+ * (host memory, in no particular order); *count (may be NULL) receives their
+ * number. */
+int ct_lc_read(ct_box_t box, ct_lc_entry* out, int64_t cap, int64_t* count);
+
 /* ---- active range and rungs ----
  * A box keeps an active range [rung_begin, N): tree builds, kicks and drifts
  * cover it (particles_current_range, src/particles.cpp:1020-1025).  It starts

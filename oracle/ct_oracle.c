@@ -13,6 +13,16 @@
  *   fixed32 / distance / drift ......... closed forms and special cases (wrap
  *                                          at 0 and 1, differences across 2^31,
  *                                          truncation of 2^32 x).
+ *   HEALPix nested pixels ................ pix2vec/vec2pix round trip of every
+ *                                          pixel, the 12 base-pixel centres,
+ *                                          the nested hierarchy (pixel at nside
+ *                                          = pixel at 2 nside / 4), equal
+ *                                          areas (counts of uniform directions).
+ *   lightcone crossing ................... each entry lies on the light shell
+ *                                          (|X(t)| = t_i - tau0 - t), a radial
+ *                                          mover against its closed form, and
+ *                                          crossings counted by a bisection of
+ *                                          the shell condition.
  *   gsoft / self_phi ..................... closed form: f(1)=phi(1)=1, f=-2P',
  *                                          Ferrers n=4 density integrates to 1,
  *                                          self_phi = 693/256.
@@ -1640,4 +1650,186 @@ void oc_direct_ewald(int64_t n, const uint32_t* x, const uint32_t* y, const uint
 	}
 	free(Sc);
 	free(Ss);
+}
+
+/* ------------------------------------------------------------------ */
+/* HEALPix nested pixels (src/healpix.cu:36-172, the HEALPix C        */
+/* library's vec2pix_nest; pix2vec_nest is its published inverse)      */
+/* ------------------------------------------------------------------ */
+
+#define HP_NS_MAX 8192
+
+/* bits of i spread to the even bit positions (the x2pix table of
+ * src/healpix.cu:36-72; y2pix = 2 x2pix) */
+static int64_t hp_spread(int64_t i) {
+	int64_t k = 0;
+	for (int b = 0; b < 32; b++) k |= ((i >> b) & 1) << (2 * b);
+	return k;
+}
+
+static int64_t hp_compress(int64_t k) {
+	int64_t i = 0;
+	for (int b = 0; b < 32; b++) i |= ((k >> (2 * b)) & 1) << b;
+	return i;
+}
+
+int64_t oc_vec2pix_nest(int64_t nside, const double vec[3]) {
+	if (nside < 1 || nside > HP_NS_MAX) return -1;
+	const double piover2 = 0.5 * M_PI, twopi = 2.0 * M_PI;
+	const double z = vec[2] / sqrt(vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2]);
+	double phi = 0.0;
+	if (vec[0] != 0.0 || vec[1] != 0.0) {
+		phi = atan2(vec[1], vec[0]);
+		if (phi < 0.0) phi += twopi;
+	}
+	const double za = fabs(z);
+	const double tt = phi / piover2;
+	int face, ix, iy;
+	if (za <= 2.0 / 3.0) {
+		/* equatorial region: edge lines of the faces */
+		const int jp = (int) floor(HP_NS_MAX * (0.5 + tt - z * 0.75));
+		const int jm = (int) floor(HP_NS_MAX * (0.5 + tt + z * 0.75));
+		const int ifp = jp / HP_NS_MAX, ifm = jm / HP_NS_MAX;
+		if (ifp == ifm) {
+			face = ifp % 4 + 4;
+		} else if (ifp < ifm) {
+			face = ifp % 4;
+		} else {
+			face = ifm % 4 + 8;
+		}
+		ix = jm % HP_NS_MAX;
+		iy = HP_NS_MAX - jp % HP_NS_MAX - 1;
+	} else {
+		/* polar caps */
+		int ntt = (int) floor(tt);
+		if (ntt >= 4) ntt = 3;
+		const double tp = tt - ntt;
+		const double tmp = sqrt(3.0 * (1.0 - za));
+		int jp = (int) floor(HP_NS_MAX * tp * tmp);
+		int jm = (int) floor(HP_NS_MAX * (1.0 - tp) * tmp);
+		if (jp > HP_NS_MAX - 1) jp = HP_NS_MAX - 1;
+		if (jm > HP_NS_MAX - 1) jm = HP_NS_MAX - 1;
+		if (z >= 0) {
+			face = ntt;
+			ix = HP_NS_MAX - jm - 1;
+			iy = HP_NS_MAX - jp - 1;
+		} else {
+			face = ntt + 8;
+			ix = jp;
+			iy = jm;
+		}
+	}
+	/* nested index at nside 8192, degraded by integer division */
+	const int64_t ipf = hp_spread(ix) + 2 * hp_spread(iy);
+	const int64_t r = HP_NS_MAX / nside;
+	return ipf / (r * r) + (int64_t) face * nside * nside;
+}
+
+/* centre of nested pixel ipix: ring index from the face's corner ring and the
+ * in-face coordinates, then z and phi of the ring (HEALPix pix2ang_nest) */
+void oc_pix2vec_nest(int64_t nside, int64_t ipix, double vec[3]) {
+	static const int jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
+	static const int jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};
+	const int64_t npface = nside * nside;
+	const int face = (int) (ipix / npface);
+	const int64_t ipf = ipix % npface;
+	const int64_t ix = hp_compress(ipf), iy = hp_compress(ipf >> 1);
+	const int64_t jr = (int64_t) jrll[face] * nside - ix - iy - 1;
+	int64_t nr, kshift;
+	double z;
+	if (jr < nside) {
+		nr = jr;
+		z = 1.0 - (double) (nr * nr) / (3.0 * (double) npface);
+		kshift = 0;
+	} else if (jr > 3 * nside) {
+		nr = 4 * nside - jr;
+		z = -(1.0 - (double) (nr * nr) / (3.0 * (double) npface));
+		kshift = 0;
+	} else {
+		nr = nside;
+		z = (double) (2 * nside - jr) * 2.0 / (3.0 * (double) nside);
+		kshift = (jr - nside) & 1;
+	}
+	int64_t jp = ((int64_t) jpll[face] * nr + ix - iy + 1 + kshift) / 2;
+	if (jp > 4 * nr) jp -= 4 * nr;
+	if (jp < 1) jp += 4 * nr;
+	const double phi = ((double) jp - (double) (kshift + 1) * 0.5) * (0.5 * M_PI / (double) nr);
+	const double st = sqrt((1.0 - z) * (1.0 + z));
+	vec[0] = st * cos(phi);
+	vec[1] = st * sin(phi);
+	vec[2] = z;
+}
+
+/* ------------------------------------------------------------------ */
+/* lightcone crossings of the drift (src/drift.cu:36-131)              */
+/* ------------------------------------------------------------------ */
+
+/* fixed<int,31> of a double: truncation of 2^31 x (cosmictiger/fixed.hpp:97-100) */
+static int32_t lc_fixed31(double x) {
+	return (int32_t) (2147483648.0 * x);
+}
+
+int64_t oc_lightcone(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, const float* vel, const int8_t* rungs, int rung,
+		float a_, float dt_, float tau0_, float tau1_, float taumax_, int nside, oc_lc_entry* out, int64_t cap) {
+	/* the float arguments of cuda_drift (cosmictiger/drift.hpp:54), in double
+	 * inside the kernel */
+	const double a = a_, dt = dt_, tau0 = tau0_, tau1 = tau1_, tau_max = taumax_;
+	if (!(tau_max - tau1 < 1.0)) return 0;
+	const double c0 = 1.0 / tau_max;
+	const double ainv = 1.0 / a;
+	const double c1 = dt / a;
+	int64_t cnt = 0;
+	for (int64_t i = 0; i < n; i++) {
+		if (rungs[i] != rung) continue;
+		const uint32_t* P[3] = {x + i, y + i, z + i};
+		double p0[3], p1[3];
+		for (int d = 0; d < 3; d++) {
+			p0[d] = fixed_to_double(*P[d]);
+			p1[d] = fma((double) vel[3 * i + d], c1, p0[d]); /* the drift's own sum, before the wrap */
+		}
+		for (int xi = -1; xi <= 0; xi++)
+			for (int yi = -1; yi <= 0; yi++)
+				for (int zi = -1; zi <= 0; zi++) {
+					const double X0[3] = {p0[0] + xi, p0[1] + yi, p0[2] + zi};
+					const double X1[3] = {p1[0] + xi, p1[1] + yi, p1[2] + zi};
+					const double dist0 = sqrt(X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2]);
+					const double dist1 = sqrt(X1[0] * X1[0] + X1[1] * X1[1] + X1[2] * X1[2]);
+					const int i0 = (int) ((tau0 + dist0) * c0);
+					const int i1 = (int) ((tau1 + dist1) * c0);
+					if (i0 == i1 || !(dist1 <= 1.0 || dist0 <= 1.0)) continue;
+					const float vx = (float) (vel[3 * i + 0] * ainv);
+					const float vy = (float) (vel[3 * i + 1] * ainv);
+					const float vz = (float) (vel[3 * i + 2] * ainv);
+					const double ti = (double) (i0 + 1) * tau_max;
+					const double A = 1.0 - (double) (vx * vx + vy * vy + vz * vz);
+					const double B = 2.0 * ((tau0 - ti) - ((double) vx * X0[0] + (double) vy * X0[1] + (double) vz * X0[2]));
+					const double C = (ti - tau0) * (ti - tau0) - (X0[0] * X0[0] + X0[1] * X0[1] + X0[2] * X0[2]);
+					const double t = -(B + sqrt(B * B - 4.0 * A * C)) / (2.0 * A);
+					const double Xc[3] = {X0[0] + (double) vx * t, X0[1] + (double) vy * t, X0[2] + (double) vz * t};
+					const double R2 = Xc[0] * Xc[0] + Xc[1] * Xc[1] + Xc[2] * Xc[2];
+					if (!(R2 <= 1.0)) continue;
+					if (cnt < cap) {
+						oc_lc_entry e;
+						e.x = lc_fixed31(Xc[0]);
+						e.y = lc_fixed31(Xc[1]);
+						e.z = lc_fixed31(Xc[2]);
+						e.vx = vel[3 * i + 0];
+						e.vy = vel[3 * i + 1];
+						e.vz = vel[3 * i + 2];
+						e.pix = (int32_t) oc_vec2pix_nest(nside, Xc);
+						e.t = t;
+						out[cnt] = e;
+					}
+					cnt++;
+				}
+	}
+	return cnt;
+}
+
+void oc_vec2pix_nest_many(int64_t nside, int64_t n, const double* vec, int64_t* out) {
+	for (int64_t i = 0; i < n; i++) out[i] = oc_vec2pix_nest(nside, vec + 3 * i);
+}
+
+void oc_pix2vec_nest_many(int64_t nside, int64_t n, const int64_t* pix, double* vec) {
+	for (int64_t i = 0; i < n; i++) oc_pix2vec_nest(nside, pix[i], vec + 3 * i);
 }

@@ -140,6 +140,31 @@ int oc_kick_sample(const oc_tree* t, const oc_kick_params* p, const uint32_t* x,
 		int nthreads, int64_t sample_b, int64_t sample_e);
 void oc_lists_free(oc_lists* l);
 
+/* ---------------- lightcone ---------------- */
+/* HEALPix nested pixel of a direction (src/healpix.cu:75-172), and the
+ * centre of a pixel (HEALPix pix2ang_nest, for the round-trip pin). */
+int64_t oc_vec2pix_nest(int64_t nside, const double vec[3]);
+void oc_pix2vec_nest(int64_t nside, int64_t ipix, double vec[3]);
+void oc_vec2pix_nest_many(int64_t nside, int64_t n, const double* vec, int64_t* out);
+void oc_pix2vec_nest_many(int64_t nside, int64_t n, const int64_t* pix, double* vec);
+
+/* lc_entry (cosmictiger/lightcone.hpp:54-58): position on the lightcone as
+ * fixed<int,31> raw words, velocity, pixel; t is the crossing time
+ * (diagnostic). */
+typedef struct {
+	int32_t x, y, z;
+	float vx, vy, vz;
+	int32_t pix;
+	double t;
+} oc_lc_entry;
+
+/* Lightcone crossings of the drift of the particles of rung `rung`
+ * (src/drift.cu:75-131, float arguments as cuda_drift takes them); the
+ * drifted position is the drift's x0 + v c1 (one fma, before the wrap).
+ * Writes up to cap entries, returns the number found. */
+int64_t oc_lightcone(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, const float* vel, const int8_t* rungs, int rung,
+		float a, float dt, float tau0, float tau1, float tau_max, int nside, oc_lc_entry* out, int64_t cap);
+
 /* ---------------- drift ---------------- */
 /* Drift particles [0,n) with rung == rung by dt (float narrowing as the
  * reference's cuda_drift signature).  Exact integer semantics. */

@@ -109,6 +109,13 @@ def _declare(L):
     L.oc_lists_free.argtypes = [ctypes.POINTER(Lists)]
     L.oc_drift.argtypes = [ctypes.c_int64, _U32, _U32, _U32, _F, _I8, ctypes.c_int, ctypes.c_float, ctypes.c_float]
     L.oc_drift.restype = None
+    L.oc_vec2pix_nest_many.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _I64]
+    L.oc_vec2pix_nest_many.restype = None
+    L.oc_pix2vec_nest_many.argtypes = [ctypes.c_int64, ctypes.c_int64, _I64, _D]
+    L.oc_pix2vec_nest_many.restype = None
+    L.oc_lightcone.argtypes = [ctypes.c_int64, _U32, _U32, _U32, _F, _I8, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64]
+    L.oc_lightcone.restype = ctypes.c_int64
     L.oc_direct_ewald.argtypes = [ctypes.c_int64, _U32, _U32, _U32, ctypes.c_int64, _I64, ctypes.c_double, ctypes.c_double,
                                   _D, _D, _D, _D, ctypes.c_int]
     L.oc_direct_ewald.restype = None
@@ -300,6 +307,43 @@ def drift(x, y, z, vel, rung, rung_level, a, dt):
     lib().oc_drift(len(x), _p(x, _U32), _p(y, _U32), _p(z, _U32), _p(vel, _F), _p(rung, _I8), int(rung_level), float(a),
                    float(dt))
     return x, y, z
+
+
+def vec2pix_nest(nside, vecs):
+    """HEALPix nested pixel of each row of vecs (src/healpix.cu:75-172)."""
+    v = np.ascontiguousarray(np.atleast_2d(vecs), dtype=np.float64)
+    out = np.zeros(len(v), dtype=np.int64)
+    lib().oc_vec2pix_nest_many(int(nside), len(v), _p(v, _D), _p(out, _I64))
+    return out
+
+
+def pix2vec_nest(nside, pix):
+    """Unit vector of the centre of each nested pixel (HEALPix pix2ang_nest)."""
+    p = np.ascontiguousarray(np.atleast_1d(pix), dtype=np.int64)
+    out = np.zeros((len(p), 3))
+    lib().oc_pix2vec_nest_many(int(nside), len(p), _p(p, _I64), _p(out, _D))
+    return out
+
+
+# lc_entry (cosmictiger/lightcone.hpp:54-58) plus the crossing time
+LC_ENTRY = np.dtype([("x", np.int32), ("y", np.int32), ("z", np.int32), ("vx", np.float32), ("vy", np.float32), ("vz", np.float32),
+                     ("pix", np.int32), ("t", np.float64)], align=True)
+
+
+def lightcone(x, y, z, vel, rung, rung_level, a, dt, tau0, tau1, tau_max, nside):
+    """Lightcone crossings of the drift (src/drift.cu:75-131), as an array of
+    LC_ENTRY in particle order, then image order (xi, yi, zi in {-1, 0})."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    y = np.ascontiguousarray(y, dtype=np.uint32)
+    z = np.ascontiguousarray(z, dtype=np.uint32)
+    vel = np.ascontiguousarray(vel, dtype=np.float32)
+    rung = np.ascontiguousarray(rung, dtype=np.int8)
+    args = (len(x), _p(x, _U32), _p(y, _U32), _p(z, _U32), _p(vel, _F), _p(rung, _I8), int(rung_level), float(a), float(dt), float(tau0),
+            float(tau1), float(tau_max), int(nside))
+    n = lib().oc_lightcone(*args, None, 0)
+    out = np.zeros(max(n, 1), dtype=LC_ENTRY)
+    lib().oc_lightcone(*args, out.ctypes.data_as(ctypes.c_void_p), n)
+    return out[:n]
 
 
 def direct_ewald(x, y, z, sinks, h, GM, nthreads=8):

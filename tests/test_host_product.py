@@ -162,6 +162,25 @@ def test_abi_struct_layouts_match_the_header(ct, tmp_path):
             assert int(got["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
 
 
+def test_lc_entry_dtype_matches_the_header(ct, tmp_path):
+    """ct.LC_ENTRY (the numpy record the lightcone entries are read into) has
+    the layout of the header's ct_lc_entry."""
+    dt = ct.LC_ENTRY
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % ct.HEADER_PATH, "int main(void) {",
+             'printf("size %zu\\n", sizeof(ct_lc_entry));']
+    for f in dt.names:
+        lines.append('printf("%s %%zu\\n", offsetof(ct_lc_entry, %s));' % (f, f))
+    lines.append("return 0; }")
+    src = tmp_path / "lc.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "lc"
+    subprocess.check_call(["gcc", "-std=c11", "-o", str(exe), str(src)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().split("\n") if l)
+    assert int(got["size"]) == dt.itemsize
+    for f in dt.names:
+        assert int(got[f]) == dt.fields[f][1], f
+
+
 def test_time_counter_closed_form(ct):
     """inc / min_rung of the driver's integer time (cosmictiger/time.hpp:28-43):
     over one top-level step at max_rung R the counter visits 2^R sub-steps,
