@@ -777,10 +777,20 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 		};
 		for (int sg = hcp; sg >= 0; sg = uniform(A.seg[sg].next)) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
+			// the particle ranges of up to 64 CP leaves at once, one per lane,
+			// so that staging waits on no list or node load
+			int my_pb = 0, my_pe = 0;
 			for (int j = 0; j < sl; j++) {
-				const int o = uniform(A.out_cp[sb + j]);
-				int pb = uniform(B->nodes[o].pbegin);
-				const int pe = uniform(B->nodes[o].pend);
+				if ((j & 63) == 0) {
+					my_pb = my_pe = 0;
+					if (j + lane < sl) {
+						const DNode& on = B->nodes[A.out_cp[sb + j + lane]];
+						my_pb = on.pbegin;
+						my_pe = on.pend;
+					}
+				}
+				int pb = __builtin_amdgcn_readlane(my_pb, j & 63);
+				const int pe = __builtin_amdgcn_readlane(my_pe, j & 63);
 				while (pb < pe) {
 					const int m = min(kCellTile - tn, pe - pb);
 					for (int t = lane; t < m; t += 64) {
