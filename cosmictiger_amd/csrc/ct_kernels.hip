@@ -25,6 +25,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -221,6 +222,35 @@ __device__ __forceinline__ WalkCounts walk_count(const DNode* __restrict__ nodes
 	return w;
 }
 
+// The fields of a box descriptor the walk reads, in scalar registers.
+struct WalkBox {
+	const DNode* nodes;
+	int32_t* head;
+	int32_t nnodes;
+	float theta_inv, h;
+};
+
+__device__ __forceinline__ uint64_t uniform64(int lo, int hi) {
+	return uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(lo))) | (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(hi))) << 32);
+}
+
+__device__ __forceinline__ WalkBox load_walk_box(const BoxDesc* p) {
+	static_assert(offsetof(BoxDesc, nodes) == 0 && offsetof(BoxDesc, head) % 16 == 0 && offsetof(BoxDesc, nnodes) % 16 == 0 &&
+					offsetof(BoxDesc, theta_inv) == offsetof(BoxDesc, nnodes) + 8 && offsetof(BoxDesc, h) == offsetof(BoxDesc, nnodes) + 12,
+			"BoxDesc layout read by load_walk_box");
+	const int4* q = reinterpret_cast<const int4*>(p);
+	const int4 a = q[0];
+	const int4 b = q[offsetof(BoxDesc, head) / 16];
+	const int4 c = q[offsetof(BoxDesc, nnodes) / 16];
+	WalkBox w;
+	w.nodes = reinterpret_cast<const DNode*>(uniform64(a.x, a.y));
+	w.head = reinterpret_cast<int32_t*>(uniform64(b.x, b.y));
+	w.nnodes = __builtin_amdgcn_readfirstlane(c.x);
+	w.theta_inv = __int_as_float(__builtin_amdgcn_readfirstlane(c.z));
+	w.h = __int_as_float(__builtin_amdgcn_readfirstlane(c.w));
+	return w;
+}
+
 // reservation slots of one entry
 enum { R_E = 0, R_CC, R_CP, R_PC, R_PP, R_ACT, R_DP, R_EP, R_SEG, R_NUM };
 constexpr int kWalkBatch = 64;
@@ -256,23 +286,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 	const int bsz = max(1, min(kWalkBatch, (nact + nwaves - 1) / nwaves));
 	for (int b0 = wave * bsz; b0 < nact; b0 += nwaves * bsz) {
 		const int nb = min(bsz, nact - b0);
+		// the batch's entries, entry e in lane e, and the descriptor of their
+		// box when they share one (the usual case): the loop over entries then
+		// waits on no entry or descriptor load
+		Act my_act = {};
+		if (lane < nb) my_act = act[b0 + lane];
+		const int box0 = uniform(my_act.box);
+		const bool one_box = __ballot(lane < nb && my_act.box != box0) == 0ull;
+		const WalkBox W0 = load_walk_box(A.boxes + box0);
 		// ---- pass 1: per-entry reservations, entry e's in lane e
 		int my[R_NUM];
 #pragma unroll
 		for (int q = 0; q < R_NUM; q++) my[q] = 0;
 		int my_nx = 0, my_lf = 0, my_skip = 1;
 		for (int e = 0; e < nb; e++) {
-			const Act ae = act[b0 + e];
-			const int self = uniform(ae.self);
-			const BoxDesc Bv = load_uniform(A.boxes + uniform(ae.box));
-			const BoxDesc* B = &Bv;
-			const DNode* nodes = B->nodes;
+			const int self = __builtin_amdgcn_readlane(my_act.self, e);
+			const int edb = __builtin_amdgcn_readlane(my_act.db, e), edn = __builtin_amdgcn_readlane(my_act.dn, e);
+			const int eeb = __builtin_amdgcn_readlane(my_act.eb, e), een = __builtin_amdgcn_readlane(my_act.en, e);
+			const WalkBox W = one_box ? W0 : load_walk_box(A.boxes + __builtin_amdgcn_readlane(my_act.box, e));
+			const DNode* nodes = W.nodes;
 			const DNode s = nodes[self];
 			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
 			const bool sleaf = s.flags & 1;
-			const WalkCounts w = walk_count(nodes, B->theta_inv, B->h, s, dp, ep, A.dcat, A.ecat, uniform(ae.db), uniform(ae.dn), uniform(ae.eb),
-					uniform(ae.en), lane);
-			nchecks += uniform(ae.dn) + uniform(ae.en);
+			const WalkCounts w = walk_count(nodes, W.theta_inv, W.h, s, dp, ep, A.dcat, A.ecat, edb, edn, eeb, een, lane);
+			nchecks += edn + een;
 			if (w.efail || (sleaf && w.enext)) {
 				if (lane == 0) atomicOr(&C->walk_error, 1);
 				continue;
@@ -344,15 +381,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 		// ---- pass 2: writes
 		for (int e = 0; e < nb; e++) {
 			if (__builtin_amdgcn_readlane(my_skip, e)) continue;
-			const Act ae = act[b0 + e];
-			const int self = uniform(ae.self);
-			const int box = uniform(ae.box);
-			const BoxDesc Bv = load_uniform(A.boxes + box);
-			const BoxDesc* B = &Bv;
-			const DNode* nodes = B->nodes;
-			const float thinv = B->theta_inv, hs = B->h;
-			const int db = uniform(ae.db), dn = uniform(ae.dn);
-			const int eb = uniform(ae.eb), en = uniform(ae.en);
+			const int self = __builtin_amdgcn_readlane(my_act.self, e);
+			const int box = __builtin_amdgcn_readlane(my_act.box, e);
+			const WalkBox W = one_box ? W0 : load_walk_box(A.boxes + box);
+			const DNode* nodes = W.nodes;
+			const int db = __builtin_amdgcn_readlane(my_act.db, e), dn = __builtin_amdgcn_readlane(my_act.dn, e);
+			const int eb = __builtin_amdgcn_readlane(my_act.eb, e), en = __builtin_amdgcn_readlane(my_act.en, e);
 			const DNode s = nodes[self];
 			const bool sleaf = s.flags & 1;
 			int off[R_NUM], cnt[R_NUM];
@@ -481,7 +515,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 						Seg sg;
 						sg.begin = offs[q];
 						sg.len = cnts[q];
-						int32_t* hp = B->head + int64_t(kinds[q]) * B->nnodes + self;
+						int32_t* hp = W.head + int64_t(kinds[q]) * W.nnodes + self;
 						sg.next = *hp;
 						sg.pad = 0;
 						A.seg[sid] = sg;

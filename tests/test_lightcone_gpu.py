@@ -63,3 +63,29 @@ def test_off_outside_the_last_unit_and_argument_checks(gpu, oc):
     with pytest.raises(ct.CTError):
         box.drift_lightcone(0, 1.0, 0.02, 0.68, 0.7, 1.5, 12)
     box.close()
+
+
+def test_only_the_active_range_crosses(gpu, oc):
+    """After particles_sort_by_rung the lightcone, like the drift, covers only
+    the active range [rung_begin, N) (src/drift.cu:170-177)."""
+    ct = gpu
+    n = 30000
+    x, y, z, vel, _ = _shell_box(n, 6)
+    rung = (np.arange(n) % 3).astype(np.int8)  # rungs 0, 1, 2
+    box = ct.Box(x, y, z, vel, rung)
+    rb = box.sort_by_rung(1)
+    assert rb == n - np.count_nonzero(rung >= 1)
+    dt, tau_max = 0.02, 1.5
+    tau1 = tau_max - 0.8
+    # rung 0 lies outside the active range: no crossings (and no drift)
+    assert oc.lightcone(x, y, z, vel, rung, 0, 1.0, dt, tau1 - dt, tau1, tau_max, 16).size > 0
+    assert len(box.drift_lightcone(0, 1.0, dt, tau1 - dt, tau1, tau_max, 16)) == 0
+    got = box.drift_lightcone(1, 1.0, dt, tau1 - dt, tau1, tau_max, 16)
+    # the oracle over the rung-1 particles only: rung-0 particles are outside
+    # the range and rung-2 ones are not this rung
+    want = oc.lightcone(x, y, z, vel, rung, 1, 1.0, dt, tau1 - dt, tau1, tau_max, 16)
+    assert len(got) == len(want) > 0
+    g, w = _sorted(got), _sorted(want)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.array_equal(g[f], w[f]), f
+    box.close()
