@@ -274,6 +274,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 	const int lane = lane_id();
 	const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
 	const int nwaves = (gridDim.x * blockDim.x) >> 6;
+	__shared__ DNode s_nodes[4][kWalkBatch];
+	DNode* s_node = s_nodes[threadIdx.x >> 6];
 	const Act* act = A.act[k & 1];
 	const int32_t* dp = A.dp[k & 1];
 	const int32_t* ep = A.ep[k & 1];
@@ -294,6 +296,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 		const int box0 = uniform(my_act.box);
 		const bool one_box = __ballot(lane < nb && my_act.box != box0) == 0ull;
 		const WalkBox W0 = load_walk_box(A.boxes + box0);
+		// and the entries' sink nodes, staged in LDS (entry e's at s_node[e];
+		// the wave's own slice: a wave barrier orders the lanes' writes and
+		// the uniform reads below)
+		{
+			if (lane < nb) {
+				const DNode* ln = one_box ? W0.nodes : A.boxes[my_act.box].nodes;
+				s_node[lane] = ln[my_act.self];
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		}
 		// ---- pass 1: per-entry reservations, entry e's in lane e
 		int my[R_NUM];
 #pragma unroll
@@ -305,7 +319,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 			const int eeb = __builtin_amdgcn_readlane(my_act.eb, e), een = __builtin_amdgcn_readlane(my_act.en, e);
 			const WalkBox W = one_box ? W0 : load_walk_box(A.boxes + __builtin_amdgcn_readlane(my_act.box, e));
 			const DNode* nodes = W.nodes;
-			const DNode s = nodes[self];
+			const DNode s = s_node[e];
 			if (s.pend - s.pbegin == 0) continue;  // no sinks: nothing to classify (src/kick.cu:268)
 			const bool sleaf = s.flags & 1;
 			const WalkCounts w = walk_count(nodes, W.theta_inv, W.h, s, dp, ep, A.dcat, A.ecat, edb, edn, eeb, een, lane);
@@ -387,7 +401,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 			const DNode* nodes = W.nodes;
 			const int db = __builtin_amdgcn_readlane(my_act.db, e), dn = __builtin_amdgcn_readlane(my_act.dn, e);
 			const int eb = __builtin_amdgcn_readlane(my_act.eb, e), en = __builtin_amdgcn_readlane(my_act.en, e);
-			const DNode s = nodes[self];
+			const DNode s = s_node[e];
 			const bool sleaf = s.flags & 1;
 			int off[R_NUM], cnt[R_NUM];
 #pragma unroll
