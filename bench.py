@@ -490,8 +490,9 @@ def main():
         try:
             with open(args.traffic_file) as f:
                 tr = json.load(f)
-            if tr.get("n") == n and tr.get("workload") == args.workload and tr.get("bucket") == args.bucket:
-                traffic = tr.get("bytes_per_launch")
+            for e in tr.get("entries", [tr]):
+                if e.get("n") == n and e.get("workload") == args.workload and e.get("bucket") == args.bucket:
+                    traffic = e.get("bytes_per_launch")
         except Exception:
             traffic = None
     gpu_forces = None

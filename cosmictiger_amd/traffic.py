@@ -78,11 +78,20 @@ def main(argv=None):
     total = res["fetch"]["pp_bytes_per_kick"] + res["write"]["pp_bytes_per_kick"]
     out = {"n": a.n, "workload": a.workload, "bucket": a.bucket, "kernel": "k_pp (all leaf-size classes of one kick)",
            "bytes_per_launch": total, "fetch": res["fetch"], "write": res["write"], "command": a.command}
-    s = json.dumps(out, indent=1)
     if a.out:
+        # the file holds one entry per (n, workload, bucket): this one replaces
+        # an earlier measurement of the same case
+        entries = []
+        if os.path.exists(a.out):
+            with open(a.out) as fh:
+                old = json.load(fh)
+            entries = old.get("entries", [old] if "bytes_per_launch" in old else [])
+        entries = [e for e in entries if (e.get("n"), e.get("workload"), e.get("bucket")) != (a.n, a.workload, a.bucket)]
+        entries.append(out)
+        entries.sort(key=lambda e: (e.get("workload", ""), e.get("n", 0), e.get("bucket", 0)))
         with open(a.out, "w") as fh:
-            fh.write(s + "\n")
-    print(s)
+            fh.write(json.dumps({"entries": entries}, indent=1) + "\n")
+    print(json.dumps(out, indent=1))
     return 0
 
 
