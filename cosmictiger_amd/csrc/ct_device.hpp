@@ -87,6 +87,13 @@ struct __align__(256) HotCounter {
 	int32_t pad[63];
 };
 
+// A 64-bit event counter alone in its 256-byte block.
+constexpr int kStatSlots = 32;
+struct __align__(256) StatSlot {
+	unsigned long long v;
+	unsigned long long pad[31];
+};
+
 // Counter block in device memory (int32 unless noted).
 struct Counters {
 	int32_t act[kMaxIter + 2];
@@ -97,6 +104,11 @@ struct Counters {
 	int32_t overflow;   // bit k set: buffer k overflowed
 	int32_t walk_error;
 	unsigned long long stat[8];  // event counters of the kick
+	// event counters added by every wave of the walk, spread over slots on
+	// their own cache lines (one address for all waves serializes their
+	// atomics): checks classified, and PC (sink, multipole) pairs
+	StatSlot walk_checks[kStatSlots];
+	StatSlot walk_pc[kStatSlots];
 };
 
 struct WalkArgs {

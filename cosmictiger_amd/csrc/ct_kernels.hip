@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 	Act* nact_out = A.act[(k + 1) & 1];
 	int32_t* ndp = A.dp[(k + 1) & 1];
 	int32_t* nep = A.ep[(k + 1) & 1];
-	unsigned long long nchecks = 0;
+	unsigned long long nchecks = 0, npcs = 0;
 	// entries per batch: enough to give every wave work at this level, at most
 	// kWalkBatch (one reservation per list per batch)
 	const int bsz = max(1, min(kWalkBatch, (nact + nwaves - 1) / nwaves));
@@ -324,6 +324,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 			const bool sleaf = s.flags & 1;
 			const WalkCounts w = walk_count(nodes, W.theta_inv, W.h, s, dp, ep, A.dcat, A.ecat, edb, edn, eeb, een, lane);
 			nchecks += edn + een;
+			// PC pairs (leaf sinks only) as the M2P kernel will run them:
+			// every sink of the leaf with every multipole
+			if (sleaf) npcs += (unsigned long long) w.pc * (unsigned long long) (s.pend - s.pbegin);
 			if (w.efail || (sleaf && w.enext)) {
 				if (lane == 0) atomicOr(&C->walk_error, 1);
 				continue;
@@ -540,7 +543,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWalkWaves)
 			}
 		}
 	}
-	if (lane == 0 && nchecks) atomicAdd(&C->stat[7], nchecks);
+	if (lane == 0) {
+		if (nchecks) atomicAdd(&C->walk_checks[wave % kStatSlots].v, nchecks);
+		if (npcs) atomicAdd(&C->walk_pc[wave % kStatSlots].v, npcs);
+	}
 }
 
 struct InitArgs {
@@ -1269,13 +1275,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 	const uint32_t* py = B->y;
 	const uint32_t* pz = B->z;
 	const int hpc = uniform(B->head[int64_t(kPC) * B->nnodes + self]);
-	unsigned long long npc = 0;
 	if (hpc >= 0) {
 		for (int k = lane; k < 4 * nsink; k += 64) acc[k] = Acc(0);
 		__syncthreads();
 		for (int sg = hpc; sg >= 0; sg = uniform(A.seg[sg].next)) {
 			const int sb = uniform(A.seg[sg].begin), sl = uniform(A.seg[sg].len);
-			npc += sl;
 			const int npair = nsink * sl;
 			for (int t = lane; t < npair; t += 64) {
 				const int j = t / nsink;
@@ -1400,8 +1404,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 		part[5] = ym;
 		part[6] = zm;
 		part[7] = nm;
-		if (npc) atomicAdd(&A.cnt->stat[4], npc * (unsigned long long) nsink);
-		atomicAdd(&A.cnt->stat[6], (unsigned long long) nsink);
 	}
 }
 
@@ -1829,6 +1831,7 @@ struct KickCtx {
 	DBuf<BoxDesc> desc;
 	DBuf<int32_t> node_off, part_off;
 	int32_t total_nodes = 0, total_parts = 0;
+	int64_t total_sinks = 0;
 	int max_depth = 0;
 	DBuf<int32_t> lvl_pref, lvl_lo;  // per level d: [(nbox + 1)] prefix and [nbox] first node
 	std::vector<int32_t> lvl_total;
@@ -1929,6 +1932,10 @@ int ctx_prepare(KickCtx* c) {
 	}
 	c->total_nodes = noff[nb];
 	c->total_parts = poff[nb];
+	// the sinks of a kick: the particles of every box's active range, which
+	// its tree's leaves hold
+	c->total_sinks = 0;
+	for (int b = 0; b < nb; b++) c->total_sinks += c->boxes[b]->hp.n - c->boxes[b]->rung_begin;
 	c->max_depth = maxd;
 	// levels 1..maxd of every box
 	std::vector<int32_t> pref((maxd + 1) * (nb + 1), 0), lo((maxd + 1) * std::max(nb, 1), 0);
@@ -2296,10 +2303,15 @@ int ctx_finish(KickCtx* c, hipStream_t st, ct_kick_return* rets, bool events) {
 	c->stats[6] = double(C.stat[1]);
 	c->stats[7] = double(C.stat[2]);
 	c->stats[8] = double(C.stat[3]);
-	c->stats[9] = double(C.stat[4]);
+	unsigned long long checks = 0, pcs = 0;
+	for (int q = 0; q < kStatSlots; q++) {
+		checks += C.walk_checks[q].v;
+		pcs += C.walk_pc[q].v;
+	}
+	c->stats[9] = double(pcs);
 	c->stats[10] = double(c->total_nodes);
-	c->stats[11] = double(C.stat[6]);
-	c->stats[14] = double(C.stat[7]);
+	c->stats[11] = double(c->total_sinks);
+	c->stats[14] = double(checks);
 	return CT_OK;
 }
 

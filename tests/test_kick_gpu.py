@@ -61,6 +61,30 @@ def test_kick_matches_oracle(gpu, oc, kind, n, theta, bucket):
     for f in ("kin", "dkin", "nmom"):
         a, b = getattr(r, f), getattr(o, f)
         assert abs(a - b) <= 1e-4 * max(abs(b), 1e-30), (f, a, b)
+    _check_event_counts(res, n)
+
+
+def _check_event_counts(res, n):
+    """The kick's event counters (the roofline's pair counts among them) equal
+    the counts of the oracle's interaction lists."""
+    st = res["stats"]
+    size = (res["onodes"]["pend"] - res["onodes"]["pbegin"]).astype(np.int64)
+    want = {}
+    for kind, key in ((0, "ewald_cc"), (1, "cc")):
+        off, idx = res["olists"][kind]
+        want[key] = int(off[-1])
+    off, idx = res["olists"][2]  # CP: every particle of every listed source leaf
+    want["cp_parts"] = int(size[idx[:off[-1]]].sum())
+    off, idx = res["olists"][3]  # PC: every sink of the leaf with every multipole
+    want["pc_pairs"] = int((np.diff(off) * size[:len(off) - 1]).sum())
+    off, idx = res["olists"][4]  # PP: sinks x sources
+    csum = np.concatenate([[0], np.cumsum(size[idx[:off[-1]]])])
+    src = csum[off[1:]] - csum[off[:-1]]  # source particles per sink node
+    want["pp_pairs"] = int((src * size[:len(off) - 1]).sum())
+    want["sinks"] = n
+    for k, v in want.items():
+        assert int(st[k]) == v, (k, st[k], v)
+    assert st["checks"] > 0
 
 
 def test_kick_nonzero_velocities_and_rungs(gpu, oc):
