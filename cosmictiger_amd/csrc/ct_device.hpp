@@ -109,6 +109,10 @@ struct Counters {
 	// atomics): checks classified, and PC (sink, multipole) pairs
 	StatSlot walk_checks[kStatSlots];
 	StatSlot walk_pc[kStatSlots];
+	// the same for the counts of the kernels with a workgroup per leaf or
+	// node: P2P pairs and P2L (CP) particles
+	StatSlot pp_pairs[kStatSlots];
+	StatSlot cp_parts[kStatSlots];
 };
 
 struct WalkArgs {

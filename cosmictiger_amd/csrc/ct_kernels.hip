@@ -865,7 +865,7 @@ __global__ __launch_bounds__(64) void k_cells_direct(CellArgs A) {
 	ncp = wave_sum_u64(ncp);
 	if (lane == 0) {
 		if (ncc) atomicAdd(&A.cnt->stat[1], ncc);
-		if (ncp) atomicAdd(&A.cnt->stat[3], ncp);
+		if (ncp) atomicAdd(&A.cnt->cp_parts[blockIdx.x % kStatSlots].v, ncp);
 	}
 }
 
@@ -1244,7 +1244,7 @@ __global__ __launch_bounds__(64) void k_pp(PPArgs A) {
 			}
 		}
 	}
-	if (lane == 0) atomicAdd(&A.cnt->stat[0], (unsigned long long) nsrc * (unsigned long long) nsink);
+	if (lane == 0) atomicAdd(&A.cnt->pp_pairs[blockIdx.x % kStatSlots].v, (unsigned long long) nsrc * (unsigned long long) nsink);
 }
 
 // M2P (src/gravity.cu:157-211), then L2P and the kick (src/kick.cu:53-177):
@@ -2299,15 +2299,17 @@ int ctx_finish(KickCtx* c, hipStream_t st, ct_kick_return* rets, bool events) {
 	}
 	CT_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[5]));
 	c->stats[4] = ms;
-	c->stats[5] = double(C.stat[0]);
-	c->stats[6] = double(C.stat[1]);
-	c->stats[7] = double(C.stat[2]);
-	c->stats[8] = double(C.stat[3]);
-	unsigned long long checks = 0, pcs = 0;
+	unsigned long long checks = 0, pcs = 0, pps = 0, cps = 0;
 	for (int q = 0; q < kStatSlots; q++) {
 		checks += C.walk_checks[q].v;
 		pcs += C.walk_pc[q].v;
+		pps += C.pp_pairs[q].v;
+		cps += C.cp_parts[q].v;
 	}
+	c->stats[5] = double(pps);
+	c->stats[6] = double(C.stat[1]);
+	c->stats[7] = double(C.stat[2]);
+	c->stats[8] = double(cps);
 	c->stats[9] = double(pcs);
 	c->stats[10] = double(c->total_nodes);
 	c->stats[11] = double(c->total_sinks);
