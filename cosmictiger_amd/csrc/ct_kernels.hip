@@ -3,19 +3,21 @@
 // Kick = five stages, all on one stream, no host synchronisation inside the
 // walk:
 //   1. k_walk (one launch per tree level): the dual-list walk of
-//      src/kick.cu:239-484, done level-synchronously.  One wave64 per sink
-//      node classifies its direct and Ewald check lists (two passes: count,
-//      then write in order), reserves output space with one atomic per list,
-//      and emits the interaction lists plus the check lists of the next level.
-//      Leaves that still hold unresolved checks stay in the work set, which is
-//      the reference's do { } while (checks && leaf) loop (src/kick.cu:344-416).
+//      src/kick.cu:239-484, done level-synchronously.  A wave takes a batch
+//      of (sink node, check lists) entries, classifies their direct and Ewald
+//      check lists (two passes: count, then write in order), reserves the
+//      batch's output space with one atomic per list, and emits the
+//      interaction lists plus the check lists of the next level.  Leaves that
+//      still hold unresolved checks stay in the work set, which is the
+//      reference's do { } while (checks && leaf) loop (src/kick.cu:344-416).
 //   2. k_cells_ewald / k_cells_direct: one wave per node, a source per lane,
 //      M2L over its Ewald and direct CC lists and P2L over its CP list
 //      (src/gravity.cu:44-155,213-244), then a wave reduction of L.
 //   3. k_l2l: per level, L(node) += L2L(L(parent)) (src/kick.cu:246-258).
-//   4. k_leaf: one wave64 per leaf: P2P with sources staged in LDS, M2P, L2P
-//      and the kick update (src/gravity.cu:157-211,246-404; src/kick.cu:53-177).
-//   5. k_reduce: deterministic FP64 reduction of the per-leaf kick_return.
+//   4. k_pp (per leaf-size class) and k_kick: one wave64 per leaf: P2P with
+//      sources staged in LDS; then M2P, L2P and the kick update
+//      (src/gravity.cu:157-211,246-404; src/kick.cu:53-177).
+//   5. k_reduce_*: deterministic FP64 reduction of the per-leaf kick_return.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
